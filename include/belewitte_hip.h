@@ -1,0 +1,363 @@
+/*
+ * belewitte_hip.h -- C ABI of the MI355X (gfx950) graph-kernel library,
+ * libbelewitte_hip.so.
+ *
+ * Plain pointers, sizes and status codes only.  Every function returns
+ * BW_HIP_OK (0) on success or a non-zero status; bw_hip_last_error() then
+ * describes the failure (thread-local).  Pointers documented as "device"
+ * must come from bw_hip_dev_alloc (or any hipMalloc'ed memory of the current
+ * device); "host" pointers are ordinary or pinned host memory.  A `stream`
+ * is a hipStream_t passed as void*; NULL selects the null stream.
+ *
+ * Reference interfaces replaced (paths relative to merijn/Belewitte):
+ *   - CUDA.hpp:30-288, CUDA.cpp:28-241 (CUDABackend: device query, pinned
+ *     host + device allocations, copies, launch configuration) and
+ *     utils/cuda_utils.hpp:1-18 (CUDA_CHK, memcpy_SIMD) -> bw_hip_* below.
+ *   - Backend.cpp:52-70 (computeDivision) -> bw_hip_compute_division.
+ *   - the bfs/ kernel files and bfs/bfs.cu (level kernels, frontier
+ *     counter) -> bw_bfs_*.
+ *   - the pagerank/ kernel files and pagerank/pagerank.cu (iteration,
+ *     degree and consolidate kernels, diff counter) -> bw_pr_*.
+ * Each declaration names the reference item it replaces.
+ */
+#ifndef BELEWITTE_HIP_H
+#define BELEWITTE_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------ status */
+
+#define BW_HIP_OK 0
+#define BW_HIP_ERR_ARG 1      /* rejected on the host before any launch   */
+#define BW_HIP_ERR_RUNTIME 2  /* a HIP runtime call failed                */
+#define BW_HIP_ERR_NODEVICE 3 /* no GPU visible                           */
+#define BW_HIP_ERR_NOMEM 4    /* device or pinned allocation failed       */
+#define BW_HIP_ERR_LIMIT 5    /* a level / iteration bound was exceeded   */
+
+/* Message of the last failure on this thread (CUDA.cpp:7-26 cudaAssert). */
+const char *bw_hip_last_error(void);
+
+/* Version string of the kernel library (KERNEL_COMMIT, Algorithm.hpp:8). */
+const char *bw_hip_version(void);
+
+/* ----------------------------------------------------------- backend */
+
+/* Device properties (CUDA.cpp:55-156 queryDevice, CUDA.cpp:158-193). */
+typedef struct bw_hip_props {
+    char name[256];
+    char arch[64];                 /* gcnArchName, e.g. "gfx950:sramecc+:xnack-" */
+    int32_t compute_units;
+    int32_t wave_size;             /* 64 on CDNA                              */
+    int32_t max_threads_per_block;
+    int32_t max_block_dims[3];
+    int32_t max_grid_dims[3];
+    uint64_t shared_mem_per_block; /* LDS bytes a block may allocate          */
+    uint64_t total_global_mem;
+    uint64_t l2_cache_bytes;
+    int32_t clock_khz;
+    int32_t memory_clock_khz;
+    int32_t memory_bus_width_bits;
+    int32_t pad_;
+} bw_hip_props;
+
+/* CUDABackend ctor, CUDA.cpp:197-217: *count = 0 and BW_HIP_OK when no
+ * device is visible. */
+int bw_hip_device_count(int32_t *count);
+/* CUDA.cpp:158-193 setDevice */
+int bw_hip_set_device(int32_t device);
+int bw_hip_get_device_props(int32_t device, bw_hip_props *props);
+
+/* cuda_alloc_t allocHostPtr / allocDevPtr, CUDA.hpp:34-61 */
+int bw_hip_dev_alloc(void **ptr, size_t bytes);
+int bw_hip_dev_free(void *ptr);
+int bw_hip_host_alloc(void **ptr, size_t bytes); /* pinned */
+int bw_hip_host_free(void *ptr);
+
+/* copyHostToDevImpl / copyDevToHostImpl, CUDA.hpp:68-90.  Synchronous:
+ * the copy is complete on return (as cudaMemcpy). */
+int bw_hip_memcpy_h2d(void *dst_device, const void *src_host, size_t bytes);
+int bw_hip_memcpy_d2h(void *dst_host, const void *src_device, size_t bytes);
+int bw_hip_memcpy_d2d(void *dst_device, const void *src_device, size_t bytes);
+/* Stream-ordered copy of any direction (hipMemcpyDefault). */
+int bw_hip_memcpy_async(void *dst, const void *src, size_t bytes, void *stream);
+
+/* Fill with a kernel (not hipMemset), so it is safe inside graph capture:
+ * dst[i] = value for i < count, 32-bit elements. */
+int bw_hip_fill_u32(void *dst_device, uint32_t value, uint64_t count, void *stream);
+
+int bw_hip_device_sync(void);
+int bw_hip_stream_create(void **stream); /* default priority */
+int bw_hip_stream_destroy(void *stream);
+int bw_hip_stream_sync(void *stream);
+int bw_hip_event_create(void **event);
+int bw_hip_event_destroy(void *event);
+int bw_hip_event_record(void *event, void *stream);
+int bw_hip_event_sync(void *event);
+int bw_hip_event_elapsed_ms(float *ms, void *start, void *end);
+
+/* Launch geometry of Backend.cpp:52-70 with the block size rounded to a
+ * multiple of 64 (wave64) instead of 32. */
+int bw_hip_compute_division(uint64_t count, uint64_t *threads_per_block,
+                            uint64_t *blocks);
+
+/* Streaming copy kernel of n 32-bit words (dst[i] = src[i]); used to
+ * calibrate the HBM byte counters (FETCH_SIZE / WRITE_SIZE). */
+int bw_hip_copy_kernel(uint32_t *dst_device, const uint32_t *src_device,
+                       uint64_t n, void *stream);
+
+/* ------------------------------------------------------- launch config */
+
+/* Launch configuration of one kernel (setWorkSizes, CUDA.cpp:195-241, plus
+ * the -w/-c warp options of ImplementationTemplate.hpp:431-436).
+ * block/grid = 0 lets the library choose; block must be a multiple of 64
+ * and at most 1024.  warp is the virtual warp width of the _warp variants
+ * (power of two, 1..64: wave64 replaces the reference's 32 limit); chunk is
+ * a multiple of warp. */
+typedef struct bw_launch {
+    uint32_t block;
+    uint32_t grid;
+    uint32_t warp;
+    uint32_t chunk;
+    void *stream;
+} bw_launch;
+
+/* Frontier-count reductions of bfs/bfs.hpp:32-116 (name suffixes "",
+ * "-bulk", "-warpreduce", "-blockreduce"). */
+#define BW_RED_NORMAL 0 /* one atomic per discovered vertex          */
+#define BW_RED_BULK 1   /* per-thread count, one atomic per thread    */
+#define BW_RED_WAVE 2   /* wave64 reduce, one atomic per wave         */
+#define BW_RED_BLOCK 3  /* wave reduce + LDS block reduce, one atomic */
+
+/* --------------------------------------------------------------- BFS */
+
+/* Level kernels.  Each call launches one level: for every vertex at
+ * `depth` the kernel discovers its unvisited neighbours, sets them to
+ * depth+1 and adds the number of newly discovered vertices to *frontier
+ * (device uint32, zeroed by the caller; resetFrontier/getFrontier of
+ * bfs/bfs.cu:7-18 become bw_hip_fill_u32 / bw_hip_memcpy_d2h).  levels is
+ * the device int32 depth array, INT_MAX meaning unvisited (bfs/bfs.cpp:71).
+ * CSR offsets are uint32 or uint64 (offset_bytes 4 or 8); ids are uint32.
+ * EdgeList arrays are the SoA (in, out) of GraphRep.hpp:17-25, StructEdgeList
+ * the AoS (in, out) pairs of GraphRep.hpp:27-38, both in the order
+ * GraphLoader.hpp:134-143 builds them; the reverse forms are built from the
+ * reverse CSR (in = destination, out = source). */
+
+/* edgeListBfs<R>, bfs/edgelist.cu:3-22 */
+int bw_bfs_edge_list(uint64_t edge_count, const uint32_t *in_edges,
+                     const uint32_t *out_edges, int32_t *levels, int32_t depth,
+                     uint32_t *frontier, int32_t reduction, const bw_launch *cfg);
+/* revEdgeListBfs<R>, bfs/rev_edgelist.cu:3-21 */
+int bw_bfs_rev_edge_list(uint64_t edge_count, const uint32_t *in_edges,
+                         const uint32_t *out_edges, int32_t *levels,
+                         int32_t depth, uint32_t *frontier, int32_t reduction,
+                         const bw_launch *cfg);
+/* structEdgeListBfs<R>, bfs/struct_edgelist.cu:3-22 */
+int bw_bfs_struct_edge_list(uint64_t edge_count, const uint32_t *edges,
+                            int32_t *levels, int32_t depth, uint32_t *frontier,
+                            int32_t reduction, const bw_launch *cfg);
+/* revStructEdgeListBfs<R>, bfs/rev_struct_edgelist.cu:3-21 */
+int bw_bfs_rev_struct_edge_list(uint64_t edge_count, const uint32_t *edges,
+                                int32_t *levels, int32_t depth,
+                                uint32_t *frontier, int32_t reduction,
+                                const bw_launch *cfg);
+/* vertexPushBfs<R>, bfs/vertex_push.cu:3-26 (forward CSR) */
+int bw_bfs_vertex_push(uint64_t vertex_count, const void *offsets,
+                       int32_t offset_bytes, const uint32_t *cols,
+                       int32_t *levels, int32_t depth, uint32_t *frontier,
+                       int32_t reduction, const bw_launch *cfg);
+/* vertexPullBfs<R>, bfs/vertex_pull.cu:3-31 (reverse CSR) */
+int bw_bfs_vertex_pull(uint64_t vertex_count, const void *rev_offsets,
+                       int32_t offset_bytes, const uint32_t *rev_cols,
+                       int32_t *levels, int32_t depth, uint32_t *frontier,
+                       int32_t reduction, const bw_launch *cfg);
+/* vertexPushWarpBfs<R>, bfs/vertex_push_warp.cu:4-60 (forward CSR) */
+int bw_bfs_vertex_push_warp(uint64_t vertex_count, const void *offsets,
+                            int32_t offset_bytes, const uint32_t *cols,
+                            int32_t *levels, int32_t depth, uint32_t *frontier,
+                            int32_t reduction, const bw_launch *cfg);
+/* vertexPullWarpBfs<R>, bfs/vertex_pull_warp.cu:4-71 (reverse CSR) */
+int bw_bfs_vertex_pull_warp(uint64_t vertex_count, const void *rev_offsets,
+                            int32_t offset_bytes, const uint32_t *rev_cols,
+                            int32_t *levels, int32_t depth, uint32_t *frontier,
+                            int32_t reduction, const bw_launch *cfg);
+
+/* Device-resident representations of one graph, as GraphLoader.hpp:296-303
+ * holds them.  Unused members may be NULL; a run fails with BW_HIP_ERR_ARG
+ * when a scheduled variant needs a missing one. */
+typedef struct bw_graph_dev {
+    uint64_t vertex_count;
+    uint64_t edge_count;
+    const void *offsets;      /* forward CSR offsets [V+1]              */
+    const uint32_t *cols;     /* forward CSR columns [E]                */
+    const void *rev_offsets;  /* reverse CSR offsets [V+1]              */
+    const uint32_t *rev_cols; /* reverse CSR columns [E]                */
+    const uint32_t *el_in;    /* forward EdgeList in[]  (source)        */
+    const uint32_t *el_out;   /* forward EdgeList out[] (destination)   */
+    const uint32_t *rel_in;   /* reverse EdgeList in[]  (destination)   */
+    const uint32_t *rel_out;  /* reverse EdgeList out[] (source)        */
+    const uint32_t *sel;      /* forward StructEdgeList (in,out) pairs  */
+    const uint32_t *rsel;     /* reverse StructEdgeList (in,out) pairs  */
+    int32_t offset_bytes;     /* 4 or 8, for both CSR directions        */
+    int32_t pad_;
+} bw_graph_dev;
+
+/* BFS variant ids (the 8 kernel families of bfs/bfs.cpp:121-207, then the
+ * queue-based forms added for MI355X). */
+#define BW_BFS_EDGE_LIST 0
+#define BW_BFS_REV_EDGE_LIST 1
+#define BW_BFS_STRUCT_EDGE_LIST 2
+#define BW_BFS_REV_STRUCT_EDGE_LIST 3
+#define BW_BFS_VERTEX_PUSH 4
+#define BW_BFS_VERTEX_PULL 5
+#define BW_BFS_VERTEX_PUSH_WARP 6
+#define BW_BFS_VERTEX_PULL_WARP 7
+#define BW_BFS_NUM_REFERENCE_VARIANTS 8
+
+/* One launch of variant `variant` (dispatch over the kernels above). */
+int bw_bfs_level(const bw_graph_dev *g, int32_t variant, int32_t reduction,
+                 int32_t *levels, int32_t depth, uint32_t *frontier,
+                 const bw_launch *cfg);
+
+/* Expands the EdgeList / StructEdgeList of GraphLoader.hpp:123-145 from a
+ * device CSR on the device: in[j] = row of j, out[j] = cols[j]; pairs gets
+ * the interleaved (in, out) form.  Any output may be NULL. */
+int bw_build_edge_lists(uint64_t vertex_count, uint64_t edge_count,
+                        const void *offsets, int32_t offset_bytes,
+                        const uint32_t *cols, uint32_t *in_out,
+                        uint32_t *out_out, uint32_t *pairs_out, void *stream);
+
+/* The level loop of bfs/bfs.cpp:67-107: levels = INT_MAX, levels[root] = 0,
+ * then per level reset the counter, launch schedule[level % len], read the
+ * counter back (a host round trip per level, as the reference does), stop
+ * when no vertex was discovered.  counts_host[k] receives the count of launch
+ * k and level_ms_host[k] (optional) its device time; *num_levels the number
+ * of launches.  root >= V leaves levels untouched and reports 0 levels
+ * (bfs/bfs.cpp:53).  BW_HIP_ERR_LIMIT if max_levels launches did not finish. */
+int bw_bfs_run(const bw_graph_dev *g, const int32_t *schedule,
+               int32_t schedule_len, int32_t reduction, uint32_t root,
+               int32_t *levels, uint32_t *counts_host, float *level_ms_host,
+               int64_t max_levels, int64_t *num_levels, const bw_launch *cfg);
+
+/* ---------------------------------------------------------- PageRank */
+
+/* Degree kernels, pagerank/pagerank.cu:19-123.  deg is device uint32[V]. */
+/* zeroInitDegreesKernel :19-26 */
+int bw_pr_zero_degrees(uint64_t vertex_count, uint32_t *deg, const bw_launch *cfg);
+/* edgeListComputeDegrees :40-51 (origins = EdgeList in[]) and
+ * reverseEdgeListComputeDegrees :79-91 (origins = reverse EdgeList out[]) */
+int bw_pr_degrees_edge_list(uint64_t edge_count, const uint32_t *origins,
+                            uint32_t *deg, const bw_launch *cfg);
+/* structEdgeListComputeDegrees :53-64 (reverse = 0, counts .in) and
+ * reverseStructEdgeListComputeDegrees :93-104 (reverse = 1, counts .out) */
+int bw_pr_degrees_struct_edge_list(uint64_t edge_count, const uint32_t *edges,
+                                   int32_t reverse, uint32_t *deg,
+                                   const bw_launch *cfg);
+/* CSRComputeDegrees :66-77 */
+int bw_pr_degrees_csr(uint64_t vertex_count, const void *offsets,
+                      int32_t offset_bytes, uint32_t *deg, const bw_launch *cfg);
+/* reverseCSRComputeDegrees :106-123 */
+int bw_pr_degrees_rev_csr(uint64_t vertex_count, const void *rev_offsets,
+                          int32_t offset_bytes, const uint32_t *rev_cols,
+                          uint32_t *deg, const bw_launch *cfg);
+
+/* Iteration kernels: accumulate incoming contributions into new_pr
+ * (zeroed by the previous consolidate).  pr/new_pr are device float[V]. */
+/* edgeListPageRank, pagerank/edgelist.cu:3-23 */
+int bw_pr_edge_list(uint64_t edge_count, const uint32_t *in_edges,
+                    const uint32_t *out_edges, const uint32_t *deg,
+                    const float *pr, float *new_pr, const bw_launch *cfg);
+/* revEdgeListPageRank, pagerank/rev_edgelist.cu:3-23 */
+int bw_pr_rev_edge_list(uint64_t edge_count, const uint32_t *in_edges,
+                        const uint32_t *out_edges, const uint32_t *deg,
+                        const float *pr, float *new_pr, const bw_launch *cfg);
+/* structEdgeListPageRank, pagerank/struct_edgelist.cu:3-24 */
+int bw_pr_struct_edge_list(uint64_t edge_count, const uint32_t *edges,
+                           const uint32_t *deg, const float *pr, float *new_pr,
+                           const bw_launch *cfg);
+/* revStructEdgeListPageRank, pagerank/rev_struct_edgelist.cu:3-24 */
+int bw_pr_rev_struct_edge_list(uint64_t edge_count, const uint32_t *edges,
+                               const uint32_t *deg, const float *pr,
+                               float *new_pr, const bw_launch *cfg);
+/* vertexPushPageRank, pagerank/vertex_push.cu:3-31 (degree from the CSR) */
+int bw_pr_vertex_push(uint64_t vertex_count, const void *offsets,
+                      int32_t offset_bytes, const uint32_t *cols,
+                      const float *pr, float *new_pr, const bw_launch *cfg);
+/* vertexPullPageRank, pagerank/vertex_pull.cu:3-31 */
+int bw_pr_vertex_pull(uint64_t vertex_count, const void *rev_offsets,
+                      int32_t offset_bytes, const uint32_t *rev_cols,
+                      const uint32_t *deg, const float *pr, float *new_pr,
+                      const bw_launch *cfg);
+/* vertexPullNoDivPageRank, pagerank/vertex_pull_nodiv.cu:3-29 */
+int bw_pr_vertex_pull_nodiv(uint64_t vertex_count, const void *rev_offsets,
+                            int32_t offset_bytes, const uint32_t *rev_cols,
+                            const float *pr, float *new_pr,
+                            const bw_launch *cfg);
+/* vertexPushWarpPageRank, pagerank/vertex_push_warp.cu:4-49 */
+int bw_pr_vertex_push_warp(uint64_t vertex_count, const void *offsets,
+                           int32_t offset_bytes, const uint32_t *cols,
+                           const float *pr, float *new_pr, const bw_launch *cfg);
+/* vertexPullWarpPageRank, pagerank/vertex_pull_warp.cu:4-49 */
+int bw_pr_vertex_pull_warp(uint64_t vertex_count, const void *rev_offsets,
+                           int32_t offset_bytes, const uint32_t *rev_cols,
+                           const uint32_t *deg, const float *pr, float *new_pr,
+                           const bw_launch *cfg);
+/* vertexPullWarpNoDivPageRank, pagerank/vertex_pull_warp_nodiv.cu:4-47 */
+int bw_pr_vertex_pull_warp_nodiv(uint64_t vertex_count, const void *rev_offsets,
+                                 int32_t offset_bytes, const uint32_t *rev_cols,
+                                 const float *pr, float *new_pr,
+                                 const bw_launch *cfg);
+
+/* PageRank modes. */
+#define BW_PR_DANGLING_DROP 0    /* reference: dangling mass is lost        */
+#define BW_PR_DANGLING_UNIFORM 1 /* dangling mass redistributed uniformly   */
+
+/* consolidateRank (nodiv = 0, pagerank/pagerank.cu:125-140) and
+ * consolidateRankNoDiv (nodiv = 1, :142-165).  DROP mode keeps the
+ * reference arithmetic bit for bit (double base term for nodiv = 0, float
+ * base term with a fused multiply-add for nodiv = 1).  UNIFORM mode computes
+ * new = (float)((1-d)/V + d*(new_pr + dangling/V)) in FP64, reading the FP64
+ * dangling mass from *dangling (device).  The L1 residual |new - old| is
+ * accumulated in FP64 into *resid (device, optional): the reference's float
+ * `diff` (:4-17, :28-38).  Zeroes new_pr. */
+int bw_pr_consolidate(uint64_t vertex_count, const uint32_t *deg, float *pr,
+                      float *new_pr, int32_t not_last, int32_t nodiv,
+                      int32_t mode, const double *dangling, double *resid,
+                      const bw_launch *cfg);
+
+/* FP64 dangling mass: *out += sum of pr[v] over deg[v] == 0 (device). */
+int bw_pr_dangling_mass(uint64_t vertex_count, const uint32_t *deg,
+                        const float *pr, double *out, const bw_launch *cfg);
+
+/* PageRank variant ids (pagerank/pagerank.cpp:205-351). */
+#define BW_PR_EDGE_LIST 0
+#define BW_PR_REV_EDGE_LIST 1
+#define BW_PR_STRUCT_EDGE_LIST 2
+#define BW_PR_REV_STRUCT_EDGE_LIST 3
+#define BW_PR_VERTEX_PUSH 4
+#define BW_PR_VERTEX_PULL 5
+#define BW_PR_VERTEX_PUSH_WARP 6
+#define BW_PR_VERTEX_PULL_WARP 7
+#define BW_PR_VERTEX_PULL_NODIV 8
+#define BW_PR_VERTEX_PULL_WARP_NODIV 9
+#define BW_PR_NUM_REFERENCE_VARIANTS 10
+
+/* The run of pagerank/pagerank.cpp:86-127 for one variant: ranks = 1/V,
+ * degrees once, `iterations` x (iteration kernel + consolidate), with no
+ * host round trip inside the loop.  ranks_out (device float[V]) receives
+ * the final ranks; resid_host (optional, double[iterations]) the L1
+ * residual per iteration.  In UNIFORM mode the nodiv contributions start
+ * pre-divided (the reference's undivided first iteration is a DROP-mode
+ * quirk only). */
+int bw_pr_run(const bw_graph_dev *g, int32_t variant, int32_t iterations,
+              int32_t mode, float *ranks_out, double *resid_host,
+              const bw_launch *cfg);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
