@@ -152,6 +152,13 @@ _HIP_SIGS = {
     "bw_pr_consolidate": (C.c_int, [C.c_uint64, vp, vp, vp, C.c_int32, C.c_int32, C.c_int32, vp, vp, _L]),
     "bw_pr_dangling_mass": (C.c_int, [C.c_uint64, vp, vp, vp, _L]),
     "bw_pr_run": (C.c_int, [_G, C.c_int32, C.c_int32, C.c_int32, vp, f64p, _L]),
+    # fused pull PageRank
+    "bw_pr_plan_create": (C.c_int, [C.POINTER(vp), C.c_uint64, C.c_uint64, vp, C.c_int32, vp]),
+    "bw_pr_plan_destroy": (C.c_int, [vp]),
+    "bw_pr_plan_info": (C.c_int, [vp, u32p, u32p, u32p]),
+    "bw_pr_plan_degrees": (C.c_int, [vp, C.POINTER(vp)]),
+    "bw_pr_fused_acc_len": (C.c_uint64, [C.c_int32]),
+    "bw_pr_fused_run": (C.c_int, [_G, vp, C.c_int32, C.c_int32, C.c_int32, vp, vp, vp, f64p, f32p, _L]),
 }
 
 

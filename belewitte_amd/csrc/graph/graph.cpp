@@ -61,38 +61,85 @@ namespace {
 inline uint64_t uniform_below(uint64_t h, uint64_t n)
 { return static_cast<uint64_t>((static_cast<unsigned __int128>(h) * n) >> 64); }
 
-// Counting-sort the pairs into rows of `key`, then sort + unique each row.
-// Returns offsets/cols with rows sorted ascending and duplicates merged.
+// Sorts the pairs into rows of `key` without atomics: a stable parallel
+// partition by the key's high bits into at most 1024 contiguous row ranges,
+// then one thread per range places its pairs row by row.  Each row is then
+// sorted and deduplicated.  Returns offsets/cols with rows sorted ascending
+// and duplicates merged.
 void build_csr(uint64_t V, const std::vector<uint32_t> &key,
                const std::vector<uint32_t> &val, bool drop_self_loops,
                std::vector<uint64_t> &off, std::vector<uint32_t> &col)
 {
     const uint64_t n = key.size();
-    std::vector<uint64_t> cnt(V + 1, 0);
+    uint32_t vbits = 0;
+    while ((1ULL << vbits) < V) vbits++;
+    const uint32_t bbits = std::min<uint32_t>(10, vbits);
+    const uint32_t shift = vbits - bbits;
+    const uint64_t nb = 1ULL << bbits;
+    const int T = omp_get_max_threads();
+    const uint64_t chunk = (n + T - 1) / T;
+
+    // Pass 1: per-thread bucket histograms.
+    std::vector<uint64_t> hist(static_cast<size_t>(T) * nb, 0);
+#pragma omp parallel num_threads(T)
     {
-        auto *c = cnt.data();
-#pragma omp parallel for schedule(static)
-        for (int64_t i = 0; i < static_cast<int64_t>(n); i++) {
+        const int t = omp_get_thread_num();
+        uint64_t *h = &hist[static_cast<size_t>(t) * nb];
+        const uint64_t b0 = std::min(n, t * chunk), b1 = std::min(n, b0 + chunk);
+        for (uint64_t i = b0; i < b1; i++) {
             if (drop_self_loops && key[i] == val[i]) continue;
-            __atomic_fetch_add(&c[key[i]], 1, __ATOMIC_RELAXED);
+            h[key[i] >> shift]++;
         }
     }
-    std::vector<uint64_t> start(V + 1);
+    // Bucket-major, thread-minor positions keep the partition stable.
+    std::vector<uint64_t> bstart(nb + 1, 0);
     uint64_t acc = 0;
+    for (uint64_t b = 0; b < nb; b++) {
+        bstart[b] = acc;
+        for (int t = 0; t < T; t++) {
+            uint64_t c = hist[static_cast<size_t>(t) * nb + b];
+            hist[static_cast<size_t>(t) * nb + b] = acc;
+            acc += c;
+        }
+    }
+    bstart[nb] = acc;
+    const uint64_t m = acc;
+    std::vector<uint32_t> pk(m), pv(m);
+#pragma omp parallel num_threads(T)
+    {
+        const int t = omp_get_thread_num();
+        uint64_t *h = &hist[static_cast<size_t>(t) * nb];
+        const uint64_t b0 = std::min(n, t * chunk), b1 = std::min(n, b0 + chunk);
+        for (uint64_t i = b0; i < b1; i++) {
+            if (drop_self_loops && key[i] == val[i]) continue;
+            uint64_t pos = h[key[i] >> shift]++;
+            pk[pos] = key[i];
+            pv[pos] = val[i];
+        }
+    }
+
+    // Pass 2: row counts per bucket (rows of a bucket are disjoint).
+    std::vector<uint64_t> cnt(V + 1, 0);
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int64_t b = 0; b < static_cast<int64_t>(nb); b++)
+        for (uint64_t i = bstart[b]; i < bstart[b + 1]; i++) cnt[pk[i]]++;
+    std::vector<uint64_t> start(V + 1);
+    acc = 0;
     for (uint64_t v = 0; v < V; v++) { start[v] = acc; acc += cnt[v]; }
     start[V] = acc;
 
-    std::vector<uint32_t> tmp(acc);
-    {
-        std::vector<uint64_t> cursor(start.begin(), start.end());
-        auto *cur = cursor.data();
-#pragma omp parallel for schedule(static)
-        for (int64_t i = 0; i < static_cast<int64_t>(n); i++) {
-            if (drop_self_loops && key[i] == val[i]) continue;
-            uint64_t pos = __atomic_fetch_add(&cur[key[i]], 1, __ATOMIC_RELAXED);
-            tmp[pos] = val[i];
-        }
+    std::vector<uint32_t> tmp(m);
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int64_t b = 0; b < static_cast<int64_t>(nb); b++) {
+        const uint64_t r0 = static_cast<uint64_t>(b) << shift;
+        const uint64_t r1 = std::min<uint64_t>(V, static_cast<uint64_t>(b + 1) << shift);
+        if (r0 >= r1) continue;
+        std::vector<uint64_t> cur(start.begin() + static_cast<int64_t>(r0),
+                                  start.begin() + static_cast<int64_t>(r1));
+        for (uint64_t i = bstart[b]; i < bstart[b + 1]; i++) tmp[cur[pk[i] - r0]++] = pv[i];
     }
+    std::vector<uint32_t>().swap(pk);
+    std::vector<uint32_t>().swap(pv);
 
     // Per-row canonicalisation; cnt[v] becomes the deduplicated degree.
 #pragma omp parallel for schedule(dynamic, 4096)

@@ -8,7 +8,6 @@ PageRank with the dangling mass spread uniformly.
 from __future__ import annotations
 
 import ctypes as C
-import struct
 
 import numpy as np
 
@@ -84,3 +83,60 @@ def format_output(ranks: np.ndarray, validate: bool = False) -> str:
     if validate:
         return "".join(f"{i}\t{float(v):.7g}\n" for i, v in enumerate(ranks.tolist()))
     return "".join(f"{i}\n" for i in bucket_order(ranks).tolist())
+
+
+FUSED = "vertex-pull-nodiv-fused"
+
+
+class FusedPageRank:
+    """The MI355X-native fused pull iteration (one launch per iteration).
+
+    Holds the plan and the workspaces, so repeated runs allocate nothing."""
+
+    def __init__(self, dg: DeviceGraph, max_iterations: int = MAX_ITERATIONS):
+        if dg.offset_bytes != 4:
+            raise ValueError("the fused kernel reads uint32 offsets")
+        dg.ensure("rev_csr")
+        self.dg = dg
+        lib = N.hip_lib()
+        p = C.c_void_p()
+        N.check_hip(lib.bw_pr_plan_create(C.byref(p), dg.vertex_count, dg.edge_count,
+                                          dg.desc.rev_offsets, 4, dg.desc.rev_cols))
+        self._plan = p
+        self.max_iterations = int(max_iterations)
+        V = max(dg.vertex_count, 1)
+        self.ranks = DeviceArray(V, np.float32)
+        self.scratch = DeviceArray(V, np.float32)
+        self.acc = DeviceArray(int(lib.bw_pr_fused_acc_len(self.max_iterations)), np.float64)
+
+    def info(self) -> dict:
+        a, b, c = C.c_uint32(), C.c_uint32(), C.c_uint32()
+        N.check_hip(N.hip_lib().bw_pr_plan_info(self._plan, C.byref(a), C.byref(b), C.byref(c)))
+        return {"items": a.value, "long_rows": b.value, "long_pieces": c.value}
+
+    def run(self, iterations: int = 30, mode: str = "uniform", nodiv_quirk: bool = False,
+            download: bool = True, stream=None, grid: int = 0):
+        """Returns (ranks or None, residuals, device ms of the iteration launches)."""
+        if not 1 <= iterations <= self.max_iterations:
+            raise ValueError("iterations out of range of the workspace")
+        resid = np.zeros(iterations, dtype=np.float64)
+        ms = C.c_float(0.0)
+        cfg = N.launch(grid=grid, stream=stream)
+        N.check_hip(N.hip_lib().bw_pr_fused_run(
+            self.dg.ref, self._plan, iterations, MODES[mode], int(nodiv_quirk), self.ranks.ptr,
+            self.scratch.ptr, self.acc.ptr, resid.ctypes.data_as(N.f64p), C.byref(ms), C.byref(cfg)))
+        out = self.ranks.download()[:self.dg.vertex_count] if download else None
+        return out, resid, float(ms.value)
+
+    def close(self):
+        if self._plan:
+            N.hip_lib().bw_pr_plan_destroy(self._plan)
+            self._plan = None
+        for a in (self.ranks, self.scratch, self.acc):
+            a.free()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

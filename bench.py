@@ -1,0 +1,171 @@
+"""Benchmark of the hot path on one MI355X.
+
+Workload: PageRank on a directed R-MAT graph, scale 22 (4,194,304 vertices),
+edge factor 16 (67,108,864 generated edges, self-loops and duplicates
+removed), a/b/c/d = 0.57/0.19/0.19/0.05, seed 2; FP32 ranks, damping 0.85,
+uniform 1/N start, dangling mass redistributed uniformly, 30 iterations.
+A step is one full 30-iteration run of the fused pull iteration
+(vertex-pull-nodiv-fused), inputs resident in HBM.  value = GE/s per
+iteration = E / (step time / 30).
+
+Prints one JSON line (rank 0).  Extra fields: the HBM roofline of the
+iteration kernel measured with HIP events on its stream, the CPU baseline
+(the oracle's OpenMP pull PageRank on the host cores, the same workload),
+the error of the GPU ranks against that baseline and against the FP64
+oracle, and the per-iteration rate of each of the reference's 10 variants.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "BFS GTEPS: learned switch vs best fixed variant; PageRank GE/s/iter vs HBM"
+HBM_PEAK_GBS = 8000.0
+SCALE, EDGE_FACTOR, SEED, ITERS = 22, 16, 2, 30
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def traffic_from_profile():
+    """HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE +
+    WRITE_SIZE, calibrated on a copy kernel); see profiles/."""
+    p = ROOT / "profiles" / "pagerank_fused_rmat_s22_traffic.json"
+    if not p.exists():
+        return None
+    try:
+        return json.loads(p.read_text()).get("bytes_per_launch")
+    except (OSError, ValueError):
+        return None
+
+
+def cpu_baseline(g, ranks_gpu, threads: int):
+    from oracle import binding as O
+    csr = O.CSR.from_graph(g)
+    t0 = time.perf_counter()
+    ranks = O.pagerank_pull_omp(csr, ITERS, threads)
+    dt = time.perf_counter() - t0
+    ge_s = g.edge_count * ITERS / dt / 1e9
+    err = float(np.max(np.abs(ranks.astype(np.float64) - ranks_gpu.astype(np.float64))))
+    return {"value": round(ge_s, 4), "unit": "GE/s/iter", "cores": threads, "kind": "port",
+            "sample": f"full workload: {ITERS} iterations of the oracle's OpenMP vertex-pull-nodiv "
+                      f"(uniform dangling) over all {g.edge_count} edges, {dt:.2f} s"}, err, csr
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-variants", action="store_true", help="skip the per-variant table")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
+              "(the multi-GPU sharding is described in DESIGN.md, not run).", file=sys.stderr)
+        return 2
+
+    from belewitte_amd import device, graph, pagerank
+    device.require_gpu()
+    device.set_device(0)
+    props = device.device_props(0)
+
+    t0 = time.perf_counter()
+    g = graph.rmat(SCALE, EDGE_FACTOR, seed=SEED)
+    V, E = g.vertex_count, g.edge_count
+    log(f"graph {g} built in {time.perf_counter() - t0:.1f}s")
+    dg = device.DeviceGraph(g, reps=("rev_csr",))
+    fused = pagerank.FusedPageRank(dg, max_iterations=ITERS)
+    log(f"plan {fused.info()}")
+
+    # Warm-up, then K timed steps bracketed by device synchronisation.
+    for _ in range(args.warmup):
+        fused.run(ITERS, "uniform", download=False)
+    device.synchronize()
+    kernel_ms = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        _, _, ms = fused.run(ITERS, "uniform", download=False)
+        kernel_ms.append(ms)
+    device.synchronize()
+    wall = time.perf_counter() - t0
+    ms_per_step = wall * 1e3 / args.steps
+    iter_s = ms_per_step / 1e3 / ITERS
+    value = E / iter_s / 1e9
+
+    # Roofline of the iteration kernel: algorithmic bytes 8E + 16V + 4 per
+    # launch over the average launch duration (HIP events on its stream).
+    bytes_per_launch = 8 * E + 16 * V + 4
+    avg_launch_s = float(np.mean(kernel_ms)) / 1e3 / ITERS
+    achieved = bytes_per_launch / avg_launch_s / 1e9
+    log(f"step {ms_per_step:.3f} ms, launch avg {avg_launch_s * 1e6:.1f} us, {achieved:.0f} GB/s")
+
+    ranks, resid, _ = fused.run(ITERS, "uniform")
+    out = {
+        "metric": METRIC,
+        "value": round(value, 3),
+        "unit": "GE/s/iter",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "pagerank vertex-pull-nodiv-fused, directed R-MAT scale 22 edge factor 16 "
+                               "(a,b,c=0.57,0.19,0.19, seed 2), 30 iterations, uniform dangling",
+                   "vertices": V, "edges": E, "iterations": ITERS, "parallelism": "single-gpu"},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profile(),
+                     "algorithmic_bytes_per_launch": bytes_per_launch,
+                     "avg_launch_us": round(avg_launch_s * 1e6, 2)},
+        "device": {"arch": props["arch"], "compute_units": props["compute_units"]},
+        "final_residual": float(resid[-1]),
+    }
+
+    if not args.no_cpu:
+        threads = len(os.sched_getaffinity(0))
+        threads = min(threads, 16)
+        base, err_cpu, csr = cpu_baseline(g, ranks, threads)
+        out["cpu_baseline"] = base
+        from oracle import binding as O
+        x = O.pagerank_f64(csr, ITERS, "uniform")
+        out["accuracy"] = {"max_abs_vs_cpu_f32": err_cpu,
+                           "max_abs_vs_f64": float(np.max(np.abs(ranks - x))),
+                           "max_rel_vs_f64": float(np.max(np.abs(ranks - x) / np.maximum(x, 1e-30)))}
+        log(f"cpu baseline {base['value']} GE/s/iter, max-abs vs cpu {err_cpu:.3e}")
+
+    if not args.no_variants:
+        dg.ensure("csr")
+        table = {}
+        for name in pagerank.VARIANTS:
+            dg.ensure(pagerank.REPRESENTATION[pagerank.variant_id(name)])
+            device.synchronize()
+            t0 = time.perf_counter()
+            pagerank.run(dg, name, iterations=ITERS, mode="uniform")
+            dt = time.perf_counter() - t0
+            table[name] = round(E * ITERS / dt / 1e9, 3)
+            log(f"variant {name}: {table[name]} GE/s/iter")
+        table[pagerank.FUSED] = round(value, 3)
+        out["variants_ge_s_iter"] = table
+
+    print(json.dumps(out), flush=True)
+    fused.close()
+    dg.free()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -357,6 +357,42 @@ int bw_pr_run(const bw_graph_dev *g, int32_t variant, int32_t iterations,
               int32_t mode, float *ranks_out, double *resid_host,
               const bw_launch *cfg);
 
+/* ------------------------------------------- fused pull PageRank (MI355X) */
+
+/* "vertex-pull-nodiv-fused": the recurrence of vertexPullNoDivPageRank +
+ * consolidateRankNoDiv (pagerank/vertex_pull_nodiv.cu:3-29,
+ * pagerank/pagerank.cu:142-165) in one launch per iteration, mapped
+ * CSR-Adaptive style (LDS-staged row blocks, long rows split across
+ * workgroups), Kahan-compensated per-vertex sums, FP64 dangling mass and L1
+ * residual.  A plan (work items + out-degrees) is built once per graph from
+ * the device reverse CSR (uint32 offsets). */
+typedef struct bw_pr_plan bw_pr_plan;
+
+int bw_pr_plan_create(bw_pr_plan **plan, uint64_t vertex_count,
+                      uint64_t edge_count, const void *rev_offsets,
+                      int32_t offset_bytes, const uint32_t *rev_cols);
+int bw_pr_plan_destroy(bw_pr_plan *plan);
+int bw_pr_plan_info(const bw_pr_plan *plan, uint32_t *items,
+                    uint32_t *long_rows, uint32_t *long_pieces);
+/* Device out-degrees owned by the plan. */
+int bw_pr_plan_degrees(const bw_pr_plan *plan, const uint32_t **deg);
+
+/* Number of doubles of the `acc` workspace of bw_pr_fused_run. */
+uint64_t bw_pr_fused_acc_len(int32_t iterations);
+
+/* `iterations` fused launches on cfg->stream.  ranks (device float[V])
+ * receives the final ranks; scratch (device float[V]) and acc (device
+ * double[bw_pr_fused_acc_len]) are caller-owned workspaces, so the run
+ * allocates nothing.  nodiv_quirk != 0 in DROP mode reproduces the
+ * reference's undivided first iteration.  resid_host (optional) receives
+ * the per-iteration L1 residuals, kernel_ms (optional) the device time of
+ * the iteration launches (HIP events on cfg->stream). */
+int bw_pr_fused_run(const bw_graph_dev *g, const bw_pr_plan *plan,
+                    int32_t iterations, int32_t mode, int32_t nodiv_quirk,
+                    float *ranks, float *scratch, double *acc,
+                    double *resid_host, float *kernel_ms,
+                    const bw_launch *cfg);
+
 #ifdef __cplusplus
 }
 #endif

@@ -434,7 +434,10 @@ void oracle_pagerank_f64(const oracle_graph *g, int iterations, int mode,
         if (mode == ORACLE_PR_DANGLING_UNIFORM)
             for (uint64_t v = 0; v < V; v++)
                 if (deg[v] == 0) dangling += x[v];
-        for (uint64_t v = 0; v < V; v++) {
+        /* Each vertex sums its in-edges sequentially, so the result does
+         * not depend on the thread count. */
+#pragma omp parallel for schedule(dynamic, 1024)
+        for (int64_t v = 0; v < (int64_t)V; v++) {
             double acc = 0.0;
             for (uint64_t j = g->roff[v]; j < g->roff[v + 1]; j++) {
                 uint32_t u = g->rcol[j];

@@ -67,3 +67,49 @@ def test_bucket_order_matches_oracle(cases):
     g, csr, dg = cases[0]
     ranks, _ = pagerank.run(dg, "vertex-pull", iterations=100, mode="drop")
     assert np.array_equal(pagerank.bucket_order(ranks), O.bucket_order(ranks).astype(np.int64))
+
+
+def _hub_graph():
+    """A vertex with 20000 in-edges (three long-row pieces) plus an R-MAT."""
+    r = graph.rmat(12, 8, seed=41)
+    s, d = r.edges()
+    hub_src = np.arange(1, 20001, dtype=np.uint32) % r.vertex_count
+    s = np.concatenate([s, hub_src])
+    d = np.concatenate([d, np.zeros(hub_src.size, dtype=np.uint32)])
+    return graph.from_edges(r.vertex_count, s, d)
+
+
+@pytest.mark.parametrize("mode,quirk,iters", [("uniform", False, 30), ("drop", True, 100), ("drop", False, 20)])
+def test_fused_matches_oracle(cases, mode, quirk, iters):
+    extra = _hub_graph()
+    todo = [(g, csr, dg) for g, csr, dg in cases] + [(extra, O.CSR.from_graph(extra), device.DeviceGraph(extra))]
+    for g, csr, dg in todo:
+        print(f"case {g.name} fused mode={mode} quirk={quirk}", flush=True)
+        fp = pagerank.FusedPageRank(dg, max_iterations=iters)
+        ranks, resid, _ = fp.run(iters, mode, nodiv_quirk=quirk)
+        if mode == "drop" and not quirk:
+            # Without the quirk the drop-mode recurrence is the FP64 one.
+            x = O.pagerank_f64(csr, iters, "drop")
+            assert float(np.max(np.abs(ranks - x))) <= 1e-6
+        else:
+            want, _ = O.pagerank_f32(csr, pagerank.variant_id("vertex-pull-nodiv"), iters, mode)
+            err = float(np.max(np.abs(ranks.astype(np.float64) - want)))
+            assert err <= TOL, (g.name, mode, err)
+        x = O.pagerank_f64(csr, iters, mode, nodiv_quirk=quirk)
+        assert float(np.max(np.abs(ranks - x))) <= 1e-5 * float(np.abs(x).max()) + 1e-9
+        # Reproducible bit for bit.
+        again, _, _ = fp.run(iters, mode, nodiv_quirk=quirk)
+        assert np.array_equal(ranks.view(np.uint32), again.view(np.uint32))
+        assert np.all(np.isfinite(resid))
+        fp.close()
+    todo[-1][2].free()
+
+
+def test_fused_plan_has_long_rows(gpu):
+    g = _hub_graph()
+    dg = device.DeviceGraph(g)
+    fp = pagerank.FusedPageRank(dg, max_iterations=5)
+    info = fp.info()
+    assert info["long_rows"] >= 1 and info["long_pieces"] >= 3
+    fp.close()
+    dg.free()
