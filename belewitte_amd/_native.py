@@ -151,6 +151,7 @@ _HIP_SIGS = {
     "bw_pr_vertex_pull_warp_nodiv": (C.c_int, [C.c_uint64, vp, C.c_int32, vp, vp, vp, _L]),
     "bw_pr_consolidate": (C.c_int, [C.c_uint64, vp, vp, vp, C.c_int32, C.c_int32, C.c_int32, vp, vp, _L]),
     "bw_pr_dangling_mass": (C.c_int, [C.c_uint64, vp, vp, vp, _L]),
+    "bw_pr_predivide": (C.c_int, [C.c_uint64, vp, vp, _L]),
     "bw_pr_run": (C.c_int, [_G, C.c_int32, C.c_int32, C.c_int32, vp, f64p, _L]),
     # fused pull PageRank
     "bw_pr_plan_create": (C.c_int, [C.POINTER(vp), C.c_uint64, C.c_uint64, vp, C.c_int32, vp]),

@@ -553,6 +553,28 @@ extern "C" int bw_pr_consolidate(uint64_t V, const uint32_t *deg, float *pr, flo
     return BW_HIP_OK;
 }
 
+namespace bw {
+namespace {
+__global__ void pr_predivide_kernel(uint64_t V, const uint32_t *__restrict__ deg, float *pr)
+{
+    const uint64_t stride = static_cast<uint64_t>(blockDim.x) * gridDim.x;
+    for (uint64_t v = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; v < V; v += stride)
+        if (deg[v] != 0) pr[v] = pr[v] / static_cast<float>(deg[v]);
+}
+}  // namespace
+}  // namespace bw
+
+extern "C" int bw_pr_predivide(uint64_t V, const uint32_t *deg, float *pr, const bw_launch *cfg)
+{
+    if (V == 0) return BW_HIP_OK;
+    if (!deg || !pr) return fail(BW_HIP_ERR_ARG, "NULL device pointer");
+    uint32_t block, grid;
+    BW_TRY(resolve_geometry(cfg, V, block, grid));
+    hipLaunchKernelGGL(pr_predivide_kernel, dim3(grid), dim3(block), 0, stream_of(cfg), V, deg, pr);
+    BW_LAUNCH_CHECK();
+    return BW_HIP_OK;
+}
+
 extern "C" int bw_pr_dangling_mass(uint64_t V, const uint32_t *deg, const float *pr, double *out,
                                    const bw_launch *cfg)
 {

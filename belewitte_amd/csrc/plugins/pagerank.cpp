@@ -1,0 +1,379 @@
+// PageRank kernel library (libpagerankkernel.so): the algorithm of
+// pagerank/pagerank.cpp of merijn/Belewitte on the HIP backend.
+//
+// Registers the 10 reference variants and "switch" under `registerHIP`,
+// each a tuple (iteration kernel, consolidate, zeroInitDegrees,
+// computeDegrees), with the reference's 100 iterations and output formats.
+// Two flags extend it: --iterations N and --dangling drop|uniform (drop is
+// the reference's behaviour).
+
+#include <algorithm>
+#include <cstring>
+#include <fstream>
+#include <iomanip>
+#include <limits>
+#include <string>
+#include <vector>
+
+#include "Algorithm.hpp"
+#include "HIPBackend.hpp"
+#include "ImplementationTemplate.hpp"
+#include "Timer.hpp"
+
+namespace {
+
+// FP64 accumulators of the run: [0] L1 residual of the reference's `diff`
+// (pagerank/pagerank.cu:4-17), [1] dangling mass.
+double *accumulators()
+{
+    static double *acc = [] {
+        void *p = nullptr;
+        BW_CHK(bw_hip_dev_alloc(&p, 2 * sizeof(double)));
+        return static_cast<double *>(p);
+    }();
+    return acc;
+}
+
+int danglingMode = BW_PR_DANGLING_DROP;
+const unsigned *currentDegrees = nullptr;
+bool predividePending = false;
+
+void resetDiff()
+{
+    // Two doubles, zeroed with the 32-bit fill kernel.
+    BW_CHK(bw_hip_fill_u32(accumulators(), 0u, 4, nullptr));
+}
+
+double getDiff()
+{
+    double v[2];
+    BW_CHK(bw_hip_memcpy_d2h(v, accumulators(), sizeof v));
+    return v[0];
+}
+
+bw_launch config(size_t warp = 0, size_t chunk = 0)
+{
+    bw_launch l = HIPBackend::get().launchConfig();
+    l.warp = static_cast<uint32_t>(warp);
+    l.chunk = static_cast<uint32_t>(chunk);
+    return l;
+}
+
+// Degree kernels (pagerank/pagerank.cu:19-123).
+void zeroInitDegreesKernel(size_t count, unsigned *degrees)
+{
+    auto c = config();
+    BW_CHK(bw_pr_zero_degrees(count, degrees, &c));
+}
+
+void edgeListComputeDegrees(EdgeList<unsigned> *g, unsigned *degrees)
+{
+    auto c = config();
+    BW_CHK(bw_pr_degrees_edge_list(g->edge_count, g->inEdges, degrees, &c));
+}
+
+void reverseEdgeListComputeDegrees(EdgeList<unsigned> *g, unsigned *degrees)
+{
+    auto c = config();
+    BW_CHK(bw_pr_degrees_edge_list(g->edge_count, g->outEdges, degrees, &c));
+}
+
+void structEdgeListComputeDegrees(StructEdgeList<unsigned> *g, unsigned *degrees)
+{
+    auto c = config();
+    BW_CHK(bw_pr_degrees_struct_edge_list(g->edge_count, reinterpret_cast<const uint32_t *>(g->edges), 0, degrees, &c));
+}
+
+void reverseStructEdgeListComputeDegrees(StructEdgeList<unsigned> *g, unsigned *degrees)
+{
+    auto c = config();
+    BW_CHK(bw_pr_degrees_struct_edge_list(g->edge_count, reinterpret_cast<const uint32_t *>(g->edges), 1, degrees, &c));
+}
+
+void reverseCSRComputeDegrees(CSR<unsigned, unsigned> *g, unsigned *degrees)
+{
+    auto c = config();
+    BW_CHK(bw_pr_degrees_rev_csr(g->vertex_count, g->vertices, 4, g->edges, degrees, &c));
+}
+
+// Iteration kernels.
+void edgeListPageRank(EdgeList<unsigned> *g, unsigned *deg, float *pr, float *np)
+{
+    auto c = config();
+    BW_CHK(bw_pr_edge_list(g->edge_count, g->inEdges, g->outEdges, deg, pr, np, &c));
+}
+
+void revEdgeListPageRank(EdgeList<unsigned> *g, unsigned *deg, float *pr, float *np)
+{
+    auto c = config();
+    BW_CHK(bw_pr_rev_edge_list(g->edge_count, g->inEdges, g->outEdges, deg, pr, np, &c));
+}
+
+void structEdgeListPageRank(StructEdgeList<unsigned> *g, unsigned *deg, float *pr, float *np)
+{
+    auto c = config();
+    BW_CHK(bw_pr_struct_edge_list(g->edge_count, reinterpret_cast<const uint32_t *>(g->edges), deg, pr, np, &c));
+}
+
+void revStructEdgeListPageRank(StructEdgeList<unsigned> *g, unsigned *deg, float *pr, float *np)
+{
+    auto c = config();
+    BW_CHK(bw_pr_rev_struct_edge_list(g->edge_count, reinterpret_cast<const uint32_t *>(g->edges), deg, pr, np, &c));
+}
+
+void vertexPushPageRank(CSR<unsigned, unsigned> *g, unsigned *, float *pr, float *np)
+{
+    auto c = config();
+    BW_CHK(bw_pr_vertex_push(g->vertex_count, g->vertices, 4, g->edges, pr, np, &c));
+}
+
+void vertexPullPageRank(CSR<unsigned, unsigned> *g, unsigned *deg, float *pr, float *np)
+{
+    auto c = config();
+    BW_CHK(bw_pr_vertex_pull(g->vertex_count, g->vertices, 4, g->edges, deg, pr, np, &c));
+}
+
+void predivideOnce(uint64_t V, const unsigned *deg, float *pr)
+{
+    if (!predividePending) return;
+    predividePending = false;
+    BW_CHK(bw_pr_predivide(V, deg, pr, nullptr));
+}
+
+void vertexPullNoDivPageRank(CSR<unsigned, unsigned> *g, unsigned *deg, float *pr, float *np)
+{
+    predivideOnce(g->vertex_count, deg, pr);
+    auto c = config();
+    BW_CHK(bw_pr_vertex_pull_nodiv(g->vertex_count, g->vertices, 4, g->edges, pr, np, &c));
+}
+
+void vertexPushWarpPageRank(size_t w, size_t ch, CSR<unsigned, unsigned> *g, unsigned *, float *pr, float *np)
+{
+    auto c = config(w, ch);
+    BW_CHK(bw_pr_vertex_push_warp(g->vertex_count, g->vertices, 4, g->edges, pr, np, &c));
+}
+
+void vertexPullWarpPageRank(size_t w, size_t ch, CSR<unsigned, unsigned> *g, unsigned *deg, float *pr, float *np)
+{
+    auto c = config(w, ch);
+    BW_CHK(bw_pr_vertex_pull_warp(g->vertex_count, g->vertices, 4, g->edges, deg, pr, np, &c));
+}
+
+void vertexPullWarpNoDivPageRank(size_t w, size_t ch, CSR<unsigned, unsigned> *g, unsigned *deg, float *pr,
+                                 float *np)
+{
+    predivideOnce(g->vertex_count, deg, pr);
+    auto c = config(w, ch);
+    BW_CHK(bw_pr_vertex_pull_warp_nodiv(g->vertex_count, g->vertices, 4, g->edges, pr, np, &c));
+}
+
+// Consolidate kernels (pagerank/pagerank.cu:125-165).
+void consolidateRank(size_t V, unsigned *deg, float *pr, float *np, bool notLast)
+{
+    auto c = config();
+    const unsigned *d = danglingMode == BW_PR_DANGLING_UNIFORM ? currentDegrees : nullptr;
+    BW_CHK(bw_pr_consolidate(V, d ? d : deg, pr, np, notLast, 0, danglingMode, accumulators() + 1, accumulators(), &c));
+}
+
+void consolidateRankNoDiv(size_t V, unsigned *deg, float *pr, float *np, bool notLast)
+{
+    auto c = config();
+    BW_CHK(bw_pr_consolidate(V, deg, pr, np, notLast, 1, danglingMode, accumulators() + 1, accumulators(), &c));
+}
+
+uint32_t mask_N_bits(unsigned n) { return ~((n >= 32) ? 0xFFFFFFFFu : ((1u << n) - 1u)); }
+
+float roundPrecision(float v, uint32_t mask)
+{
+    uint32_t bits;
+    std::memcpy(&bits, &v, sizeof bits);
+    bits &= mask;
+    std::memcpy(&v, &bits, sizeof v);
+    return v;
+}
+
+}  // namespace
+
+template<typename Platform, typename Vertex, typename Edge, bool switching>
+struct PageRank : public ImplementationTemplate<Platform, Vertex, Edge, switching> {
+    using Impl = ImplementationTemplate<Platform, Vertex, Edge, switching>;
+    using Impl::backend;
+    using Impl::isSwitching;
+    using Impl::options;
+    using Impl::run_count;
+    using Impl::setKernelConfig;
+    using Impl::validate;
+    using Impl::vertex_count;
+
+    template<typename... Args>
+    using Kernel = typename Impl::template GraphKernel<Args...>;
+
+    int max_iterations = 100;  // pagerank/pagerank.cpp:55
+    std::string dangling = "drop";
+    const unsigned maskBits = 16;
+    const unsigned bucketSize = 1000;
+
+    Kernel<unsigned *> zeroInitDegrees;
+    Kernel<unsigned *> computeDegrees;
+    Kernel<unsigned *, float *, float *> kernel;
+    Kernel<unsigned *, float *, float *, bool> consolidate;
+
+    PageRank(Kernel<unsigned *, float *, float *> k, Kernel<unsigned *, float *, float *, bool> c,
+             Kernel<unsigned *> zeroInit, Kernel<unsigned *> compute)
+      : zeroInitDegrees(zeroInit), computeDegrees(compute), kernel(k), consolidate(c)
+    {
+        options.add("iterations", "NUM", max_iterations, "Number of PageRank iterations.");
+        options.add("dangling", "MODE", dangling,
+                    "Dangling mass: drop (the reference's behaviour) or uniform (redistributed).");
+    }
+
+    void runImplementation(std::ofstream &outputFile) override
+    {
+        if (dangling != "drop" && dangling != "uniform") reportError("--dangling must be drop or uniform");
+        danglingMode = dangling == "uniform" ? BW_PR_DANGLING_UNIFORM : BW_PR_DANGLING_DROP;
+
+        Timer initResults("initResults", run_count);
+        Timer pagerankTime("computation", run_count);
+        Timer pagerankStepTime("0:computation", run_count);
+        Timer resultTransfer("resultTransfer", run_count);
+
+        auto pageranks = backend.template alloc<float>(vertex_count);
+        auto new_pageranks = backend.template alloc<float>(vertex_count);
+        // UNIFORM mode needs out-degrees for every variant (the push
+        // variants have no degree kernels in the reference).
+        auto uniformDegrees = backend.template alloc<unsigned>(
+            danglingMode == BW_PR_DANGLING_UNIFORM ? vertex_count : 1);
+
+        for (size_t i = 0; i < run_count; i++) {
+            initResults.start();
+            std::fill(pageranks.begin(), pageranks.end(), 1.0f / static_cast<float>(vertex_count));
+            pageranks.copyHostToDev();
+            std::fill(new_pageranks.begin(), new_pageranks.end(), 0.0f);
+            new_pageranks.copyHostToDev();
+            initResults.stop();
+
+            pagerankTime.start();
+            pagerankStepTime.start();
+            if constexpr (isSwitching) this->predictInitial();
+
+            const uint64_t degreeBufferSize = (zeroInitDegrees || computeDegrees) ? vertex_count : 0;
+            auto degrees = backend.template alloc<unsigned>(std::max<uint64_t>(degreeBufferSize, 1));
+            if (zeroInitDegrees) setKernelConfig(zeroInitDegrees);
+            zeroInitDegrees(this->loader, degrees);
+            if (computeDegrees) setKernelConfig(computeDegrees);
+            computeDegrees(this->loader, degrees);
+
+            currentDegrees = nullptr;
+            if (danglingMode == BW_PR_DANGLING_UNIFORM) {
+                auto &fwd = this->loader.template getGraph<Rep::CSR, Dir::Forward>();
+                if (degreeBufferSize) {
+                    currentDegrees = degrees.devicePtr();
+                } else if (fwd) {
+                    BW_CHK(bw_pr_zero_degrees(vertex_count, uniformDegrees.devicePtr(), nullptr));
+                    BW_CHK(bw_pr_degrees_csr(vertex_count, fwd->vertices, 4, uniformDegrees.devicePtr(), nullptr));
+                    currentDegrees = uniformDegrees.devicePtr();
+                } else {
+                    reportError("--dangling uniform needs out-degrees this variant does not compute");
+                }
+            }
+            // UNIFORM mode starts the nodiv variants from divided
+            // contributions; their launchers apply it on the first call.
+            predividePending = danglingMode == BW_PR_DANGLING_UNIFORM;
+
+            int j = 0;
+            do {
+                j++;
+                resetDiff();
+                if (danglingMode == BW_PR_DANGLING_UNIFORM)
+                    BW_CHK(bw_pr_dangling_mass(vertex_count, currentDegrees, pageranks.devicePtr(), accumulators() + 1,
+                                               nullptr));
+                setKernelConfig(kernel);
+                kernel(this->loader, degrees, pageranks, new_pageranks);
+                setKernelConfig(consolidate);
+                consolidate(this->loader, degrees, pageranks, new_pageranks, max_iterations > j);
+                (void)getDiff();
+            } while (j < max_iterations);
+            pagerankStepTime.stop();
+            pagerankTime.stop();
+
+            resultTransfer.start();
+            pageranks.copyDevToHost();
+            resultTransfer.stop();
+        }
+
+        auto oldLocale = outputFile.imbue(std::locale::classic());
+        if (validate) {
+            outputFile << std::setprecision(std::numeric_limits<float>::digits10 + 1);
+            for (size_t i = 0; i < pageranks.size; i++) outputFile << i << "\t" << pageranks[i] << "\n";
+        } else {
+            const uint32_t mask = mask_N_bits(maskBits);
+            std::vector<std::pair<float, size_t>> buckets;
+            buckets.reserve(pageranks.size);
+            for (size_t i = 0; i < pageranks.size; i++) buckets.emplace_back(roundPrecision(pageranks[i], mask), i);
+            std::stable_sort(buckets.begin(), buckets.end());
+            for (size_t i = 0; i < buckets.size(); i += bucketSize) {
+                // The last element stays outside the last sort
+                // (pagerank/pagerank.cpp:155-156).
+                const size_t end = std::min(i + bucketSize, buckets.size() - 1);
+                if (end > i)
+                    std::stable_sort(buckets.begin() + static_cast<long>(i), buckets.begin() + static_cast<long>(end),
+                                     [](const auto &a, const auto &b) { return a.second < b.second; });
+            }
+            for (const auto &p : buckets) outputFile << p.second << "\n";
+        }
+        outputFile.imbue(oldLocale);
+    }
+};
+
+extern "C" register_algorithm_t registerHIP;
+extern "C" void registerHIP(Algorithm &result)
+{
+    INITIALISE_ALGORITHM(result);
+    KernelBuilder<HIPBackend, unsigned, unsigned> make_kernel;
+
+    auto zeroInit = make_kernel(zeroInitDegreesKernel, work_division::vertex, tag_t(Rep::VertexCount));
+    auto revCSRDegrees =
+        make_kernel(reverseCSRComputeDegrees, work_division::vertex, tag_t(Rep::CSR), tag_t(Dir::Reverse));
+    auto consolidate = make_kernel(consolidateRank, work_division::vertex, tag_t(Rep::VertexCount));
+    auto consolidateNoDiv = make_kernel(consolidateRankNoDiv, work_division::vertex, tag_t(Rep::VertexCount));
+    auto pushMem = [](size_t chunk) { return chunk * sizeof(float) + (1 + chunk) * sizeof(unsigned); };
+    auto pullMem = [](size_t chunk) { return (1 + chunk) * sizeof(unsigned); };
+    using Deg = decltype(zeroInit);
+
+    KernelMap prMap{std::pair{
+        "edge-list",
+        std::tuple{make_kernel(edgeListPageRank, work_division::edge, tag_t(Rep::EdgeList)), consolidate, zeroInit,
+                   make_kernel(edgeListComputeDegrees, work_division::edge, tag_t(Rep::EdgeList))}}};
+    prMap["rev-edge-list"] = {
+        make_kernel(revEdgeListPageRank, work_division::edge, tag_t(Rep::EdgeList), tag_t(Dir::Reverse)), consolidate,
+        zeroInit,
+        make_kernel(reverseEdgeListComputeDegrees, work_division::edge, tag_t(Rep::EdgeList), tag_t(Dir::Reverse))};
+    prMap["struct-edge-list"] = {
+        make_kernel(structEdgeListPageRank, work_division::edge, tag_t(Rep::StructEdgeList)), consolidate, zeroInit,
+        make_kernel(structEdgeListComputeDegrees, work_division::edge, tag_t(Rep::StructEdgeList))};
+    prMap["rev-struct-edge-list"] = {
+        make_kernel(revStructEdgeListPageRank, work_division::edge, tag_t(Rep::StructEdgeList), tag_t(Dir::Reverse)),
+        consolidate, zeroInit,
+        make_kernel(reverseStructEdgeListComputeDegrees, work_division::edge, tag_t(Rep::StructEdgeList),
+                    tag_t(Dir::Reverse))};
+    prMap["vertex-push"] = std::make_tuple(make_kernel(vertexPushPageRank, work_division::vertex, tag_t(Rep::CSR)),
+                                           consolidate, Deg(nullptr), Deg(nullptr));
+    prMap["vertex-pull"] = {
+        make_kernel(vertexPullPageRank, work_division::vertex, tag_t(Rep::CSR), tag_t(Dir::Reverse)), consolidate,
+        zeroInit, revCSRDegrees};
+    prMap["vertex-push-warp"] = std::make_tuple(
+        make_kernel(vertexPushWarpPageRank, work_division::vertex, pushMem, tag_t(Rep::CSR)), consolidate,
+        Deg(nullptr), Deg(nullptr));
+    prMap["vertex-pull-warp"] = {
+        make_kernel(vertexPullWarpPageRank, work_division::vertex, pullMem, tag_t(Rep::CSR), tag_t(Dir::Reverse)),
+        consolidate, zeroInit, revCSRDegrees};
+    prMap["vertex-pull-nodiv"] = {
+        make_kernel(vertexPullNoDivPageRank, work_division::vertex, tag_t(Rep::CSR), tag_t(Dir::Reverse)),
+        consolidateNoDiv, zeroInit, revCSRDegrees};
+    prMap["vertex-pull-warp-nodiv"] = {
+        make_kernel(vertexPullWarpNoDivPageRank, work_division::vertex, pullMem, tag_t(Rep::CSR), tag_t(Dir::Reverse)),
+        consolidateNoDiv, zeroInit, revCSRDegrees};
+
+    for (auto &[name, kernel] : prMap) result.addImplementation(name, make_implementation<PageRank>(kernel));
+    result.addImplementation("switch", make_switch_implementation<PageRank>(prMap));
+}

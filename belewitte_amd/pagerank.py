@@ -93,11 +93,29 @@ class FusedPageRank:
 
     Holds the plan and the workspaces, so repeated runs allocate nothing."""
 
+    @classmethod
+    def for_graph(cls, g, max_iterations: int = MAX_ITERATIONS, reorder: bool = True):
+        """Uploads `g` for the fused kernel.  reorder=True relabels the
+        vertices by decreasing out-degree first (the relabelling of
+        reorder-graph.cpp:40-128), which packs the most-gathered
+        contributions into few cache lines; ranks are returned in the
+        original vertex order either way."""
+        perm = None
+        if reorder:
+            g, perm = g.reorder_by_degree()
+        dg = DeviceGraph(g, reps=("rev_csr",))
+        obj = cls(dg, max_iterations)
+        obj.perm = perm
+        obj._owns_dg = True
+        return obj
+
     def __init__(self, dg: DeviceGraph, max_iterations: int = MAX_ITERATIONS):
         if dg.offset_bytes != 4:
             raise ValueError("the fused kernel reads uint32 offsets")
         dg.ensure("rev_csr")
         self.dg = dg
+        self.perm = None        # old vertex id -> id in dg, when relabelled
+        self._owns_dg = False
         lib = N.hip_lib()
         p = C.c_void_p()
         N.check_hip(lib.bw_pr_plan_create(C.byref(p), dg.vertex_count, dg.edge_count,
@@ -112,7 +130,7 @@ class FusedPageRank:
     def info(self) -> dict:
         a, b, c = C.c_uint32(), C.c_uint32(), C.c_uint32()
         N.check_hip(N.hip_lib().bw_pr_plan_info(self._plan, C.byref(a), C.byref(b), C.byref(c)))
-        return {"items": a.value, "long_rows": b.value, "long_pieces": c.value}
+        return {"tiles": a.value, "long_rows": b.value, "long_pieces": c.value}
 
     def run(self, iterations: int = 30, mode: str = "uniform", nodiv_quirk: bool = False,
             download: bool = True, stream=None, grid: int = 0):
@@ -125,7 +143,11 @@ class FusedPageRank:
         N.check_hip(N.hip_lib().bw_pr_fused_run(
             self.dg.ref, self._plan, iterations, MODES[mode], int(nodiv_quirk), self.ranks.ptr,
             self.scratch.ptr, self.acc.ptr, resid.ctypes.data_as(N.f64p), C.byref(ms), C.byref(cfg)))
-        out = self.ranks.download()[:self.dg.vertex_count] if download else None
+        out = None
+        if download:
+            out = self.ranks.download()[:self.dg.vertex_count]
+            if self.perm is not None:
+                out = out[self.perm]
         return out, resid, float(ms.value)
 
     def close(self):
@@ -134,6 +156,9 @@ class FusedPageRank:
             self._plan = None
         for a in (self.ranks, self.scratch, self.acc):
             a.free()
+        if self._owns_dg:
+            self.dg.free()
+            self._owns_dg = False
 
     def __del__(self):
         try:

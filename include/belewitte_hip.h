@@ -329,6 +329,12 @@ int bw_pr_consolidate(uint64_t vertex_count, const uint32_t *deg, float *pr,
                       int32_t mode, const double *dangling, double *resid,
                       const bw_launch *cfg);
 
+/* pr[v] /= deg[v] where deg[v] != 0: the divided starting contributions of
+ * the nodiv variants in UNIFORM mode (no reference counterpart: the
+ * reference only has the undivided DROP-mode start, pagerank.cpp:88). */
+int bw_pr_predivide(uint64_t vertex_count, const uint32_t *deg, float *pr,
+                    const bw_launch *cfg);
+
 /* FP64 dangling mass: *out += sum of pr[v] over deg[v] == 0 (device). */
 int bw_pr_dangling_mass(uint64_t vertex_count, const uint32_t *deg,
                         const float *pr, double *out, const bw_launch *cfg);
@@ -372,7 +378,7 @@ int bw_pr_plan_create(bw_pr_plan **plan, uint64_t vertex_count,
                       uint64_t edge_count, const void *rev_offsets,
                       int32_t offset_bytes, const uint32_t *rev_cols);
 int bw_pr_plan_destroy(bw_pr_plan *plan);
-int bw_pr_plan_info(const bw_pr_plan *plan, uint32_t *items,
+int bw_pr_plan_info(const bw_pr_plan *plan, uint32_t *row_tiles,
                     uint32_t *long_rows, uint32_t *long_pieces);
 /* Device out-degrees owned by the plan. */
 int bw_pr_plan_degrees(const bw_pr_plan *plan, const uint32_t **deg);

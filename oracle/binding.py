@@ -80,8 +80,11 @@ class CSR:
     @classmethod
     def from_graph(cls, g):
         if g.undirected:
-            return cls(g.vertex_count, g.offsets(), g.cols())
-        return cls(g.vertex_count, g.offsets(), g.cols(), g.offsets(True), g.cols(True))
+            c = cls(g.vertex_count, g.offsets(), g.cols())
+        else:
+            c = cls(g.vertex_count, g.offsets(), g.cols(), g.offsets(True), g.cols(True))
+        c.graph = g  # the arrays are views into the graph's memory
+        return c
 
     @property
     def ref(self):

@@ -70,10 +70,11 @@ def test_bucket_order_matches_oracle(cases):
 
 
 def _hub_graph():
-    """A vertex with 20000 in-edges (three long-row pieces) plus an R-MAT."""
-    r = graph.rmat(12, 8, seed=41)
+    """A vertex with 20000 distinct in-edges (three long-row pieces of 8192)
+    plus an R-MAT."""
+    r = graph.rmat(15, 4, seed=41)
     s, d = r.edges()
-    hub_src = np.arange(1, 20001, dtype=np.uint32) % r.vertex_count
+    hub_src = np.arange(1, 20001, dtype=np.uint32)
     s = np.concatenate([s, hub_src])
     d = np.concatenate([d, np.zeros(hub_src.size, dtype=np.uint32)])
     return graph.from_edges(r.vertex_count, s, d)
@@ -91,11 +92,17 @@ def test_fused_matches_oracle(cases, mode, quirk, iters):
             # Without the quirk the drop-mode recurrence is the FP64 one.
             x = O.pagerank_f64(csr, iters, "drop")
             assert float(np.max(np.abs(ranks - x))) <= 1e-6
-        else:
+        x = O.pagerank_f64(csr, iters, mode, nodiv_quirk=quirk)
+        if not (mode == "drop" and not quirk):
             want, _ = O.pagerank_f32(csr, pagerank.variant_id("vertex-pull-nodiv"), iters, mode)
             err = float(np.max(np.abs(ranks.astype(np.float64) - want)))
-            assert err <= TOL, (g.name, mode, err)
-        x = O.pagerank_f64(csr, iters, mode, nodiv_quirk=quirk)
+            # The reference FP32 path sums sequentially; on a hub row its own
+            # rounding error can exceed the 1e-6 bar, so the bar is widened
+            # by that error, and the compensated sums must never be less
+            # accurate than it.
+            ref_err64 = float(np.max(np.abs(want - x)))
+            assert err <= TOL + ref_err64, (g.name, mode, err, ref_err64)
+            assert float(np.max(np.abs(ranks - x))) <= ref_err64 + 1e-7, (g.name, mode)
         assert float(np.max(np.abs(ranks - x))) <= 1e-5 * float(np.abs(x).max()) + 1e-9
         # Reproducible bit for bit.
         again, _, _ = fp.run(iters, mode, nodiv_quirk=quirk)
@@ -113,3 +120,15 @@ def test_fused_plan_has_long_rows(gpu):
     assert info["long_rows"] >= 1 and info["long_pieces"] >= 3
     fp.close()
     dg.free()
+
+
+def test_fused_degree_ordered_matches_oracle(gpu):
+    g = graph.rmat(13, 16, seed=51)
+    csr = O.CSR.from_graph(g)
+    fp = pagerank.FusedPageRank.for_graph(g, max_iterations=30, reorder=True)
+    ranks, _, _ = fp.run(30, "uniform")
+    want, _ = O.pagerank_f32(csr, pagerank.variant_id("vertex-pull-nodiv"), 30, "uniform")
+    x = O.pagerank_f64(csr, 30, "uniform")
+    assert float(np.max(np.abs(ranks - want))) <= TOL + float(np.max(np.abs(want - x)))
+    assert float(np.max(np.abs(ranks - x))) <= 1e-5 * float(x.max())
+    fp.close()
