@@ -160,6 +160,31 @@ _HIP_SIGS = {
     "bw_pr_plan_degrees": (C.c_int, [vp, C.POINTER(vp)]),
     "bw_pr_fused_acc_len": (C.c_uint64, [C.c_int32]),
     "bw_pr_fused_run": (C.c_int, [_G, vp, C.c_int32, C.c_int32, C.c_int32, vp, vp, vp, f64p, f32p, _L]),
+    # device-driven BFS engine
+    "bw_bfs_engine_create": (C.c_int, [C.POINTER(vp), _G, vp, C.c_uint32]),
+    "bw_bfs_engine_destroy": (C.c_int, [vp]),
+    "bw_bfs_engine_levels": (C.c_int, [vp, C.POINTER(vp)]),
+    "bw_bfs_engine_mlp_len": (C.c_int32, [C.c_int32]),
+    "bw_bfs_engine_set_model": (C.c_int, [vp, f32p, C.c_int32]),
+    "bw_bfs_engine_set_schedule": (C.c_int, [vp, i32p, C.c_int32]),
+    "bw_bfs_engine_begin": (C.c_int, [vp, C.c_uint32, C.c_int32, C.c_int32, C.c_int32, vp]),
+    "bw_bfs_engine_level": (C.c_int, [vp, C.c_int32, C.c_int32, C.c_int32, vp]),
+    "bw_bfs_engine_force": (C.c_int, [vp, C.c_int32, vp]),
+    "bw_bfs_engine_status": (C.c_int, [vp, u32p, u32p, u32p, i32p, f32p, vp]),
+    "bw_bfs_engine_snapshot": (C.c_int, [vp, C.c_int32, vp]),
+    "bw_bfs_engine_step_forced": (C.c_int, [vp, C.c_int32, f32p, u32p, u32p, vp]),
+    "bw_bfs_engine_run": (C.c_int, [vp, C.c_uint32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
+                                    u32p, u64p, i32p, u32p, f32p, vp]),
+    "bw_bfs_engine_set_tuning": (C.c_int, [vp, C.c_uint32, C.c_uint32]),
+    "bw_bfs_engine_begin_multi": (C.c_int, [vp, u32p, C.c_uint32, C.c_int32, C.c_int32, C.c_int32, vp]),
+    "bw_bfs_engine_run_multi": (C.c_int, [vp, u32p, C.c_uint32, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
+                                          C.c_int32, u32p, u64p, i32p, u32p, f32p, vp]),
+    # variant predictor
+    "bw_mlp_params": (C.c_int32, [C.c_int32]),
+    "bw_mlp_forward": (C.c_int, [vp, C.c_int32, vp, C.c_uint64, vp, vp]),
+    "bw_mlp_grad": (C.c_int, [vp, C.c_int32, vp, vp, C.c_uint32, vp, vp, vp]),
+    "bw_mlp_train": (C.c_int, [vp, C.c_int32, vp, vp, C.c_uint32, C.c_uint32, C.c_uint32, C.c_float,
+                               C.c_uint32, vp, vp]),
 }
 
 

@@ -1,0 +1,209 @@
+"""The device-driven BFS engine (MI355X-native level loop).
+
+Computes exactly the depths and per-level frontier counts of the
+reference's BFS (bfs/bfs.cpp:43-118 of merijn/Belewitte): INT_MAX for
+unreached vertices, one entry per level with the number of newly
+discovered vertices, the last entry 0.  Each level runs one of the native
+variants (``td``, ``bu``, ``bu-wave``), picked per level on the device by
+a fixed choice, a schedule, the on-device MLP or the host; because every
+variant yields the same depths, the choice never changes the result.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _native as N
+from .device import DeviceGraph
+
+VARIANTS = ("td", "bu", "bu-wave")
+POLICIES = {"fixed": 0, "schedule": 1, "learned": 2, "host": 3}
+NUM_FEATURES = 8
+FEATURE_NAMES = ("log2_frontier", "log2_frontier_edges", "log2_unvisited", "depth",
+                 "log2_mean_degree", "log2_std_degree", "prev_variant", "log2_prev_time_ns")
+UNVISITED = 2147483647
+DEFAULT_HUB_DEGREE = 4096
+DEFAULT_WAVE_DEGREE = 32
+
+
+def variant_id(v) -> int:
+    if isinstance(v, (int, np.integer)):
+        if not 0 <= int(v) < len(VARIANTS):
+            raise KeyError(f"unknown native BFS variant {v}")
+        return int(v)
+    try:
+        return VARIANTS.index(v)
+    except ValueError:
+        raise KeyError(f"unknown native BFS variant {v!r}; known: {', '.join(VARIANTS)}") from None
+
+
+@dataclass
+class EngineResult:
+    levels: np.ndarray | None   # int32 depth per vertex (None if not downloaded)
+    counts: np.ndarray          # newly discovered vertices per level (last is 0)
+    times_ns: np.ndarray        # device time per level
+    choices: np.ndarray         # variant per level
+    total_ms: float             # device time of the whole search
+
+    @property
+    def num_levels(self) -> int:
+        return int(self.counts.size)
+
+    @property
+    def reached(self) -> int:
+        return int(self.counts.sum()) + 1
+
+
+@dataclass
+class LevelRecord:
+    """One level of an exhaustive run: its features and every variant's time."""
+    source: int
+    depth: int
+    features: np.ndarray        # float32[8]
+    ms: np.ndarray              # float64[K] device time of each variant's level kernel
+    count: int                  # vertices discovered at this level
+
+
+class BFSEngine:
+    """Holds the engine's device state for one graph (both CSR directions,
+    uint32 offsets).  Repeated searches allocate nothing."""
+
+    def __init__(self, dg: DeviceGraph, max_levels: int | None = None):
+        if dg.offset_bytes != 4:
+            raise ValueError("the BFS engine reads uint32 offsets")
+        dg.ensure("csr")
+        dg.ensure("rev_csr")
+        self.dg = dg
+        V = dg.vertex_count
+        self.max_levels = int(max_levels or min(V + 1, 1 << 22))
+        p = C.c_void_p()
+        N.check_hip(N.hip_lib().bw_bfs_engine_create(C.byref(p), dg.ref, None, self.max_levels))
+        self._e = p
+        self._counts = np.zeros(self.max_levels, dtype=np.uint32)
+        self._times = np.zeros(self.max_levels, dtype=np.uint64)
+        self._choices = np.zeros(self.max_levels, dtype=np.int32)
+        self.model_outputs = 0
+        self.tuning = (DEFAULT_HUB_DEGREE, DEFAULT_WAVE_DEGREE)
+
+    # ------------------------------------------------------------ set-up
+
+    def set_model(self, weights: np.ndarray) -> None:
+        w = np.ascontiguousarray(weights, dtype=np.float32)
+        k = len(VARIANTS)
+        if w.size != int(N.hip_lib().bw_bfs_engine_mlp_len(k)):
+            raise ValueError("model size does not match the engine's variant count")
+        N.check_hip(N.hip_lib().bw_bfs_engine_set_model(self._e, w.ctypes.data_as(N.f32p), k))
+        self.model_outputs = k
+
+    def set_schedule(self, schedule) -> int:
+        s = np.ascontiguousarray([variant_id(v) for v in schedule], dtype=np.int32)
+        N.check_hip(N.hip_lib().bw_bfs_engine_set_schedule(self._e, s.ctypes.data_as(N.i32p), s.size))
+        return int(s.size)
+
+    def set_tuning(self, hub_degree: int = DEFAULT_HUB_DEGREE, wave_degree: int = DEFAULT_WAVE_DEGREE) -> None:
+        N.check_hip(N.hip_lib().bw_bfs_engine_set_tuning(self._e, int(hub_degree), int(wave_degree)))
+        self.tuning = (int(hub_degree), int(wave_degree))
+
+    # -------------------------------------------------------------- runs
+
+    def run(self, root, policy: str = "fixed", variant="td", schedule=None, levels_per_sync: int = 8,
+            use_graph: bool = True, download: bool = True, stream=None) -> EngineResult:
+        """A whole search with the level loop on the device.  `root` may be
+        a list of roots, all at depth 0 (a multi-root search)."""
+        pol = POLICIES[policy]
+        slen = self.set_schedule(schedule) if schedule is not None else 1
+        if policy == "schedule" and schedule is None:
+            raise ValueError("the schedule policy needs a schedule")
+        n = C.c_uint32(0)
+        ms = C.c_float(0.0)
+        outs = (self._counts.ctypes.data_as(N.u32p), self._times.ctypes.data_as(N.u64p),
+                self._choices.ctypes.data_as(N.i32p), C.byref(n), C.byref(ms), stream)
+        if np.ndim(root) == 0:
+            N.check_hip(N.hip_lib().bw_bfs_engine_run(
+                self._e, int(root), pol, variant_id(variant), slen, int(levels_per_sync), int(bool(use_graph)),
+                *outs))
+        else:
+            roots = np.ascontiguousarray(root, dtype=np.uint32)
+            N.check_hip(N.hip_lib().bw_bfs_engine_run_multi(
+                self._e, roots.ctypes.data_as(N.u32p), roots.size, pol, variant_id(variant), slen,
+                int(levels_per_sync), int(bool(use_graph)), *outs))
+        k = min(int(n.value), self.max_levels)
+        return EngineResult(self.levels() if download else None, self._counts[:k].copy(),
+                            self._times[:k].copy(), self._choices[:k].copy(), float(ms.value))
+
+    def levels(self) -> np.ndarray:
+        p = C.c_void_p()
+        N.check_hip(N.hip_lib().bw_bfs_engine_levels(self._e, C.byref(p)))
+        V = self.dg.vertex_count
+        out = np.empty(V, dtype=np.int32)
+        N.check_hip(N.hip_lib().bw_hip_memcpy_d2h(out.ctypes.data_as(C.c_void_p), p, 4 * V))
+        return out
+
+    def status(self, stream=None) -> dict:
+        done, depth, nl = C.c_uint32(), C.c_uint32(), C.c_uint32()
+        choice = C.c_int32()
+        feat = np.zeros(NUM_FEATURES, dtype=np.float32)
+        N.check_hip(N.hip_lib().bw_bfs_engine_status(self._e, C.byref(done), C.byref(depth), C.byref(nl),
+                                                     C.byref(choice), feat.ctypes.data_as(N.f32p), stream))
+        return {"done": bool(done.value), "depth": int(depth.value), "num_levels": int(nl.value),
+                "choice": int(choice.value), "features": feat}
+
+    def exhaustive(self, root: int, repeats: int = 1, stream=None):
+        """Times every variant at every level from the same saved state.
+
+        The search continues along the per-level fastest variant, so the
+        features "previous variant" and "previous time" are those of the
+        per-level oracle.  Returns (levels, counts, [LevelRecord])."""
+        lib = N.hip_lib()
+        K = len(VARIANTS)
+        N.check_hip(lib.bw_bfs_engine_begin(self._e, int(root), POLICIES["host"], 0, 1, stream))
+        records, counts = [], []
+        ms, cnt, done = C.c_float(), C.c_uint32(), C.c_uint32()
+        while True:
+            st = self.status(stream)
+            if st["done"]:
+                break
+            if st["depth"] + 1 >= self.max_levels:
+                raise N.BelewitteError(5, "exhaustive BFS reached max_levels")
+            N.check_hip(lib.bw_bfs_engine_snapshot(self._e, 0, stream))
+            times = np.full(K, np.inf)
+            seen = set()
+            for r in range(repeats):
+                for k in range(K):
+                    N.check_hip(lib.bw_bfs_engine_snapshot(self._e, 1, stream))
+                    N.check_hip(lib.bw_bfs_engine_step_forced(self._e, k, C.byref(ms), C.byref(cnt),
+                                                              C.byref(done), stream))
+                    times[k] = min(times[k], float(ms.value))
+                    seen.add(int(cnt.value))
+            if len(seen) != 1:
+                raise N.BelewitteError(2, f"variants disagree on the level-{st['depth']} count: {sorted(seen)}")
+            best = int(np.argmin(times))
+            N.check_hip(lib.bw_bfs_engine_snapshot(self._e, 1, stream))
+            N.check_hip(lib.bw_bfs_engine_step_forced(self._e, best, C.byref(ms), C.byref(cnt),
+                                                      C.byref(done), stream))
+            records.append(LevelRecord(int(root), st["depth"], st["features"].copy(), times, int(cnt.value)))
+            counts.append(int(cnt.value))
+            if done.value:
+                break
+        return self.levels(), np.asarray(counts, dtype=np.uint32), records
+
+    def close(self) -> None:
+        if self._e:
+            N.hip_lib().bw_bfs_engine_destroy(self._e)
+            self._e = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def teps_edges(g, levels: np.ndarray) -> int:
+    """Edges counted by the TEPS figures: the out-degree sum over reached
+    vertices (directed), halved for undirected graphs (Graph500)."""
+    deg = np.diff(g.offsets()).astype(np.int64)
+    e = int(deg[levels != UNVISITED].sum())
+    return e // 2 if g.undirected else e

@@ -1,0 +1,1075 @@
+// Device-driven BFS engine for gfx950: queue-based top-down and bottom-up
+// level kernels, a per-level epilogue that computes the frontier features
+// and picks the next level's variant (fixed, scheduled, host-forced, or by a
+// small MLP), and a level loop that needs no host round trip per level.
+//
+// Every variant computes the level of merijn/Belewitte's BFS
+// (bfs/bfs.cpp:93-107, bfs/vertex_push.cu, bfs/vertex_pull.cu): vertices at
+// `depth` discover their unvisited neighbours, which get depth + 1; the
+// frontier count is the number of newly discovered vertices.  So every
+// per-level schedule of variants yields the same depths and counts.  The
+// variants differ only in how they find the work:
+//   TD       top-down over the frontier queue; a workgroup takes 256 queue
+//            vertices, scans their degrees in LDS and expands their edges 256
+//            at a time (edge-balanced); vertices whose out-degree exceeds
+//            the hub threshold (tunable, 4096 by default) are queued
+//            separately and their edges are sliced across all workgroups.
+//   BU       bottom-up: one thread per unvisited vertex probes its
+//            in-neighbours and stops at the first one in the frontier.
+//   BU_WAVE  bottom-up with wave64 probing (64 in-neighbours per step,
+//            ballot early exit) for in-degree above the wave threshold
+//            (tunable, 32 by default), one thread per vertex below that.
+// A search may start from several roots at once (all at depth 0), which
+// runs many small graphs stored as one disjoint union in one search.
+// Frontier membership for the bottom-up probes is a bitmap (V bits, three
+// rotating: read, written, cleared), visited-ness a fourth bitmap, so the
+// probes hit a few MiB instead of the 4-byte depth array.  Discovered
+// vertices are appended to the next frontier queue through an LDS buffer
+// (wave-aggregated LDS atomics, one global atomic per flush).
+
+#include "common.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstddef>
+#include <cstring>
+#include <vector>
+
+namespace bw {
+namespace {
+
+constexpr int kBlock = 256;
+constexpr uint32_t kLdsQueue = 2048;   // staged discoveries per workgroup
+constexpr uint32_t kHubCap = 4096;     // hubs per level (more go to the normal queue)
+constexpr int kFeatures = BW_BFS_FEATURES;
+constexpr int kHidden = 32;
+constexpr int kMaxOut = 16;
+constexpr int kVariants = BW_BFS_NATIVE_VARIANTS;
+
+enum Variant : int32_t { TD = 0, BU = 1, BU_WAVE = 2 };
+enum Policy : int32_t { FIXED = BW_POLICY_FIXED, SCHEDULE = BW_POLICY_SCHEDULE, LEARNED = BW_POLICY_LEARNED,
+                        HOST = BW_POLICY_HOST };
+
+struct BfsState {
+    uint32_t depth;     // level being processed: the frontier is at `depth`
+    uint32_t done;
+    uint32_t qcount;    // normal frontier queue size
+    uint32_t hubcount;  // hub frontier queue size
+    uint32_t qout;      // appended to the next normal queue during the level
+    uint32_t hubout;    // appended to the next hub queue (may exceed kHubCap)
+    int32_t choice;     // variant of this level
+    uint32_t num_levels;
+    int32_t prev_choice;        // variant of the previous level, -1 at level 0
+    float prev_time_ns;         // device time of the previous level
+    uint64_t visited;           // discovered so far, root included
+    uint64_t frontier_edges;    // out-degree sum of the frontier
+    double frontier_sq;         // sum of squared out-degrees of the frontier
+    uint64_t t_step;            // wall clock at the start of the level's step kernel
+};
+
+struct Graph {
+    uint64_t V;
+    uint64_t E;
+    const uint32_t *off;   // out-edges (top-down)
+    const uint32_t *col;
+    const uint32_t *roff;  // in-edges (bottom-up)
+    const uint32_t *rcol;
+};
+
+struct Buffers {
+    int32_t *levels;
+    uint32_t *q[2];
+    uint32_t *hubq[2];
+    uint64_t *hub_prefix;  // kHubCap + 1
+    uint32_t *bits[3];     // frontier bitmaps, level d reads bits[d % 3]
+    uint32_t *vis;         // visited bitmap
+    BfsState *st;
+    uint64_t *part;        // per workgroup: out-degree sum, (double) sum of squares
+    uint32_t *counts;      // per level: discovered vertices
+    uint64_t *times;       // per level: device time in ns
+    int32_t *choices;      // per level: variant
+    uint32_t max_levels;
+    uint32_t words;        // bitmap words
+    float ns_per_tick;
+    uint32_t hub_degree;   // out-degree above which a discovered vertex is a hub
+    uint32_t wave_degree;  // in-degree above which BU_WAVE probes with a whole wave
+};
+
+struct PolicyArgs {
+    int32_t policy;
+    int32_t fixed;
+    const int32_t *schedule;
+    int32_t schedule_len;
+    const float *mlp;  // packed weights, see mlp_len
+    int32_t nout;
+};
+
+__device__ __forceinline__ uint32_t out_degree(const Graph &g, uint32_t v) { return g.off[v + 1] - g.off[v]; }
+__device__ __forceinline__ bool test_bit(const uint32_t *bits, uint32_t v) { return (bits[v >> 5] >> (v & 31)) & 1u; }
+
+// --------------------------------------------------------- queue emission
+
+struct Emitter {
+    uint32_t *lbuf;
+    uint32_t *lcount;
+    uint32_t *lbase;
+    uint64_t outsum = 0;
+    double sq = 0.0;
+
+    // Called by the lanes that discovered a vertex (any subset of a wave).
+    __device__ __forceinline__ void emit(const Graph &g, const Buffers &b, uint32_t d, uint32_t w)
+    {
+        const uint32_t nq = (d & 1) ^ 1;
+        const uint32_t dw = out_degree(g, w);
+        outsum += dw;
+        sq += static_cast<double>(dw) * static_cast<double>(dw);
+        atomicOr(&b.bits[(d + 1) % 3][w >> 5], 1u << (w & 31));
+        atomicOr(&b.vis[w >> 5], 1u << (w & 31));
+        if (dw > b.hub_degree) {
+            // Slots are handed out by a counter that only grows, so every
+            // slot below min(count, kHubCap) is written exactly once.
+            const uint32_t h = atomicAdd(&b.st->hubout, 1u);
+            if (h < kHubCap) {
+                b.hubq[nq][h] = w;
+                return;
+            }
+        }
+        const uint64_t m = __ballot(1);
+        const int lane = __lane_id();
+        const int leader = __ffsll(static_cast<unsigned long long>(m)) - 1;
+        const uint32_t n = __popcll(m);
+        const uint32_t rank = __popcll(m & ((1ull << lane) - 1ull));
+        uint32_t base = 0;
+        if (lane == leader) base = atomicAdd(lcount, n);
+        base = __shfl(base, leader, kWave);
+        const uint32_t pos = base + rank;
+        if (pos < kLdsQueue) {
+            lbuf[pos] = w;
+        } else {
+            b.q[nq][atomicAdd(&b.st->qout, 1u)] = w;
+        }
+    }
+
+    // Block-uniform: moves the staged vertices to the global queue when
+    // more than `reserve` further slots might not fit.
+    __device__ __forceinline__ void flush(const Buffers &b, uint32_t d, uint32_t reserve)
+    {
+        __syncthreads();
+        const uint32_t n = min(*lcount, kLdsQueue);
+        __syncthreads();
+        if (n == 0 || n + reserve <= kLdsQueue) return;
+        if (threadIdx.x == 0) *lbase = atomicAdd(&b.st->qout, n);
+        __syncthreads();
+        const uint32_t g0 = *lbase;
+        uint32_t *q = b.q[(d & 1) ^ 1];
+        for (uint32_t i = threadIdx.x; i < n; i += kBlock) q[g0 + i] = lbuf[i];
+        __syncthreads();
+        if (threadIdx.x == 0) *lcount = 0;
+        __syncthreads();
+    }
+};
+
+__device__ __forceinline__ bool try_discover(int32_t *levels, uint32_t w, int32_t nd)
+{
+    return levels[w] == kUnvisited && atomicCAS(&levels[w], kUnvisited, nd) == kUnvisited;
+}
+
+// ------------------------------------------------------------- variants
+
+__device__ void level_td(const Graph &g, const Buffers &b, uint32_t d, Emitter &em, uint32_t *sstart,
+                         uint64_t *sscan, uint64_t *stotal)
+{
+    const int32_t nd = static_cast<int32_t>(d) + 1;
+    const uint32_t cur = d & 1;
+    const BfsState &st = *b.st;
+    const uint32_t tid = threadIdx.x;
+
+    // Hub edges, sliced over all workgroups 256 edges at a time.
+    const uint32_t nh = st.hubcount;
+    if (nh) {
+        const uint64_t H = b.hub_prefix[nh];
+        for (uint64_t c = static_cast<uint64_t>(blockIdx.x) * kBlock; c < H;
+             c += static_cast<uint64_t>(gridDim.x) * kBlock) {
+            const uint64_t k = c + tid;
+            if (k < H) {
+                uint32_t lo = 0, hi = nh;  // largest h with prefix[h] <= k
+                while (hi - lo > 1) {
+                    const uint32_t mid = (lo + hi) / 2;
+                    if (b.hub_prefix[mid] <= k) lo = mid;
+                    else hi = mid;
+                }
+                const uint32_t u = b.hubq[cur][lo];
+                const uint32_t w = g.col[g.off[u] + static_cast<uint32_t>(k - b.hub_prefix[lo])];
+                if (try_discover(b.levels, w, nd)) em.emit(g, b, d, w);
+            }
+            em.flush(b, d, kBlock);
+        }
+    }
+
+    // Normal frontier: batches of 256 queue vertices, edges expanded 256 at
+    // a time with the owner found by binary search in the LDS degree scan.
+    const uint32_t n = st.qcount;
+    for (uint32_t base = blockIdx.x * kBlock; base < n; base += gridDim.x * kBlock) {
+        const uint32_t i = base + tid;
+        uint32_t deg = 0, start = 0;
+        if (i < n) {
+            const uint32_t u = b.q[cur][i];
+            start = g.off[u];
+            deg = g.off[u + 1] - start;
+        }
+        sstart[tid] = start;
+        sscan[tid] = deg;
+        __syncthreads();
+        // Inclusive Hillis-Steele scan over the 256 degrees.
+        for (uint32_t o = 1; o < kBlock; o <<= 1) {
+            const uint64_t add = tid >= o ? sscan[tid - o] : 0;
+            __syncthreads();
+            sscan[tid] += add;
+            __syncthreads();
+        }
+        if (tid == 0) *stotal = sscan[kBlock - 1];
+        __syncthreads();
+        const uint64_t total = *stotal;
+        for (uint64_t c = 0; c < total; c += kBlock) {
+            const uint64_t k = c + tid;
+            if (k < total) {
+                uint32_t lo = 0, hi = kBlock - 1;  // first j with inclusive scan > k
+                while (lo < hi) {
+                    const uint32_t mid = (lo + hi) / 2;
+                    if (sscan[mid] > k) hi = mid;
+                    else lo = mid + 1;
+                }
+                const uint64_t before = lo ? sscan[lo - 1] : 0;
+                const uint32_t w = g.col[sstart[lo] + static_cast<uint32_t>(k - before)];
+                if (try_discover(b.levels, w, nd)) em.emit(g, b, d, w);
+            }
+            em.flush(b, d, kBlock);
+        }
+        __syncthreads();
+    }
+}
+
+__device__ void level_bu(const Graph &g, const Buffers &b, uint32_t d, Emitter &em, bool wave)
+{
+    const int32_t nd = static_cast<int32_t>(d) + 1;
+    const uint32_t *front = b.bits[d % 3];
+    const uint64_t V = g.V;
+    const int lane = __lane_id();
+    for (uint64_t base = static_cast<uint64_t>(blockIdx.x) * kBlock; base < V;
+         base += static_cast<uint64_t>(gridDim.x) * kBlock) {
+        const uint64_t v64 = base + threadIdx.x;
+        const uint32_t v = static_cast<uint32_t>(v64);
+        const bool unvisited = v64 < V && !test_bit(b.vis, v);
+        uint32_t s = 0, e = 0;
+        if (unvisited) {
+            s = g.roff[v];
+            e = g.roff[v + 1];
+        }
+        const bool big = wave && e - s > b.wave_degree;
+        if (unvisited && !big) {
+            for (uint32_t j = s; j < e; j++) {
+                if (test_bit(front, g.rcol[j])) {
+                    b.levels[v] = nd;
+                    em.emit(g, b, d, v);
+                    break;
+                }
+            }
+        }
+        if (wave) {
+            uint64_t m = __ballot(big);
+            while (m) {
+                const int l = __ffsll(static_cast<unsigned long long>(m)) - 1;
+                m &= m - 1;
+                const uint32_t vs = __shfl(s, l, kWave), ve = __shfl(e, l, kWave);
+                bool found = false;
+                for (uint32_t k0 = vs; k0 < ve && !found; k0 += kWave) {
+                    const uint32_t j = k0 + lane;
+                    const bool hit = j < ve && test_bit(front, g.rcol[j]);
+                    found = __ballot(hit) != 0;
+                }
+                if (found && lane == l) {
+                    b.levels[v] = nd;
+                    em.emit(g, b, d, v);
+                }
+            }
+        }
+        em.flush(b, d, kBlock);
+    }
+}
+
+// ---------------------------------------------------------- level kernel
+
+__global__ __launch_bounds__(kBlock) void bfs_engine_step(Graph g, Buffers b)
+{
+    __shared__ uint32_t lbuf[kLdsQueue];
+    __shared__ uint32_t lcount, lbase;
+    __shared__ uint64_t stotal;
+    __shared__ uint32_t sstart[kBlock];
+    __shared__ uint64_t sscan[kBlock];
+    __shared__ uint64_t rout[kBlock / kWave];
+    __shared__ double rsq[kBlock / kWave];
+
+    BfsState &st = *b.st;
+    if (st.done) return;
+    const uint32_t d = st.depth;
+    const int32_t variant = st.choice;
+    if (blockIdx.x == 0 && threadIdx.x == 0) st.t_step = wall_clock64();
+
+    // Clear the bitmap that the next level will write.
+    uint32_t *clear = b.bits[(d + 2) % 3];
+    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < b.words; i += gridDim.x * kBlock) clear[i] = 0;
+
+    if (threadIdx.x == 0) lcount = 0;
+    __syncthreads();
+    Emitter em{lbuf, &lcount, &lbase};
+    if (variant == TD) level_td(g, b, d, em, sstart, sscan, &stotal);
+    else level_bu(g, b, d, em, variant == BU_WAVE);
+    em.flush(b, d, kLdsQueue);  // reserve everything: moves all that is staged
+
+    // Workgroup partials of the discovered vertices' out-degrees.
+    uint64_t os = em.outsum;
+    double sq = em.sq;
+    for (int o = kWave / 2; o > 0; o >>= 1) {
+        os += __shfl_xor(os, o, kWave);
+        sq += __shfl_xor(sq, o, kWave);
+    }
+    const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+    if (lane == 0) {
+        rout[wid] = os;
+        rsq[wid] = sq;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint64_t a = 0;
+        double q = 0.0;
+        for (int w = 0; w < kBlock / kWave; w++) {
+            a += rout[w];
+            q += rsq[w];
+        }
+        b.part[2 * blockIdx.x] = a;
+        b.part[2 * blockIdx.x + 1] = static_cast<uint64_t>(__double_as_longlong(q));
+    }
+}
+
+// ------------------------------------------------------------- the policy
+
+// Packed MLP: mean[8] std[8] W1[32][8] b1[32] W2[32][32] b2[32] W3[K][32] b3[K].
+__host__ __device__ constexpr int mlp_len(int K)
+{ return 2 * kFeatures + kHidden * kFeatures + kHidden + kHidden * kHidden + kHidden + K * kHidden + K; }
+
+// The 8 features of the pending level: frontier size, frontier out-edges,
+// unvisited vertices, depth, mean and standard deviation of the frontier
+// out-degree, previous variant, previous level time (counts and times as
+// log2(1 + x)).
+__host__ __device__ void features(const BfsState &st, uint64_t V, float *f)
+{
+    const double n = static_cast<double>(st.qcount) + static_cast<double>(st.hubcount);
+    const double edges = static_cast<double>(st.frontier_edges);
+    const double mean = n > 0 ? edges / n : 0.0;
+    const double var = n > 0 ? fmax(st.frontier_sq / n - mean * mean, 0.0) : 0.0;
+    f[0] = static_cast<float>(log2(1.0 + n));
+    f[1] = static_cast<float>(log2(1.0 + edges));
+    f[2] = static_cast<float>(log2(1.0 + static_cast<double>(V - st.visited)));
+    f[3] = static_cast<float>(st.depth);
+    f[4] = static_cast<float>(log2(1.0 + mean));
+    f[5] = static_cast<float>(log2(1.0 + sqrt(var)));
+    f[6] = static_cast<float>(st.prev_choice);
+    f[7] = static_cast<float>(log2(1.0 + static_cast<double>(st.prev_time_ns)));
+}
+
+// One workgroup: MLP forward on the features (one output per variant, the
+// predicted log2 time), returns the argmin.
+__device__ int32_t mlp_choose(const float *w, int K, const float *f, float *h1, float *h2, float *out)
+{
+    const float *mean = w, *sd = w + kFeatures;
+    const float *W1 = sd + kFeatures, *b1 = W1 + kHidden * kFeatures;
+    const float *W2 = b1 + kHidden, *b2 = W2 + kHidden * kHidden;
+    const float *W3 = b2 + kHidden, *b3 = W3 + K * kHidden;
+    const int t = threadIdx.x;
+    if (t < kHidden) {
+        float a = b1[t];
+        for (int i = 0; i < kFeatures; i++) a += W1[t * kFeatures + i] * ((f[i] - mean[i]) / sd[i]);
+        h1[t] = fmaxf(a, 0.0f);
+    }
+    __syncthreads();
+    if (t < kHidden) {
+        float a = b2[t];
+        for (int i = 0; i < kHidden; i++) a += W2[t * kHidden + i] * h1[i];
+        h2[t] = fmaxf(a, 0.0f);
+    }
+    __syncthreads();
+    if (t < K) {
+        float a = b3[t];
+        for (int i = 0; i < kHidden; i++) a += W3[t * kHidden + i] * h2[i];
+        out[t] = a;
+    }
+    __syncthreads();
+    int32_t best = 0;
+    for (int k = 1; k < K; k++)
+        if (out[k] < out[best]) best = k;
+    return best;
+}
+
+// Block-uniform; the result is valid in every thread.
+__device__ int32_t choose(const PolicyArgs &p, const BfsState &st, const Graph &g, float *f, float *h1, float *h2,
+                          float *out)
+{
+    switch (p.policy) {
+    case SCHEDULE: return p.schedule[st.depth % static_cast<uint32_t>(p.schedule_len)];
+    case LEARNED: {
+        if (threadIdx.x == 0) features(st, g.V, f);
+        __syncthreads();
+        const int32_t c = mlp_choose(p.mlp, p.nout, f, h1, h2, out);
+        return c < kVariants ? c : p.fixed;
+    }
+    case HOST: return st.choice;
+    default: return p.fixed;
+    }
+}
+
+// The per-level epilogue: one workgroup.  Records the finished level,
+// advances the state to the next frontier and picks its variant.
+__global__ __launch_bounds__(kBlock) void bfs_engine_epilogue(Graph g, Buffers b, PolicyArgs p, uint32_t nparts)
+{
+    __shared__ uint64_t rout[kBlock / kWave];
+    __shared__ double rsq[kBlock / kWave];
+    __shared__ uint64_t hscan[kBlock];
+    __shared__ float f[kFeatures], h1[kHidden], h2[kHidden], out[kMaxOut];
+
+    BfsState &st = *b.st;
+    if (st.done) return;
+    const uint64_t now = wall_clock64();
+    const int tid = threadIdx.x, lane = tid & (kWave - 1), wid = tid / kWave;
+    uint64_t os = 0;
+    double sq = 0.0;
+    for (uint32_t i = tid; i < nparts; i += kBlock) {
+        os += b.part[2 * i];
+        sq += __longlong_as_double(static_cast<long long>(b.part[2 * i + 1]));
+    }
+    for (int o = kWave / 2; o > 0; o >>= 1) {
+        os += __shfl_xor(os, o, kWave);
+        sq += __shfl_xor(sq, o, kWave);
+    }
+    if (lane == 0) {
+        rout[wid] = os;
+        rsq[wid] = sq;
+    }
+    const uint32_t d = st.depth;
+    const uint32_t nhub = min(st.hubout, kHubCap);
+    const uint32_t nnew = nhub + st.qout;
+    __syncthreads();
+
+    // Hub degree prefix of the next frontier (kHubCap / kBlock hubs per
+    // thread), read by the next top-down level.
+    {
+        constexpr uint32_t kPer = kHubCap / kBlock;
+        const uint32_t nq = (d & 1) ^ 1;
+        uint64_t mine[kPer];
+        uint64_t local = 0;
+        for (uint32_t k = 0; k < kPer; k++) {
+            const uint32_t h = tid * kPer + k;
+            mine[k] = h < nhub ? out_degree(g, b.hubq[nq][h]) : 0;
+            local += mine[k];
+        }
+        hscan[tid] = local;
+        __syncthreads();
+        if (tid == 0) {
+            uint64_t run = 0;
+            for (int i = 0; i < kBlock; i++) {
+                const uint64_t x = hscan[i];
+                hscan[i] = run;
+                run += x;
+            }
+        }
+        __syncthreads();
+        uint64_t run = hscan[tid];
+        for (uint32_t k = 0; k < kPer; k++) {
+            const uint32_t h = tid * kPer + k;
+            if (h <= nhub) b.hub_prefix[h] = run;
+            run += mine[k];
+        }
+        if (tid == kBlock - 1 && nhub == kHubCap) b.hub_prefix[kHubCap] = run;
+    }
+
+    if (tid == 0) {
+        uint64_t a = 0;
+        double q = 0.0;
+        for (int w = 0; w < kBlock / kWave; w++) {
+            a += rout[w];
+            q += rsq[w];
+        }
+        // Level time: from the start of the step kernel's first workgroup
+        // to the start of this epilogue.
+        const float ns = static_cast<float>(now - st.t_step) * b.ns_per_tick;
+        if (d < b.max_levels) {
+            b.counts[d] = nnew;
+            b.times[d] = static_cast<uint64_t>(ns);
+            b.choices[d] = st.choice;
+        }
+        st.num_levels = d + 1;
+        if (nnew == 0 || d + 1 >= b.max_levels) {
+            st.done = 1;
+        } else {
+            st.prev_choice = st.choice;
+            st.prev_time_ns = ns;
+            st.depth = d + 1;
+            st.qcount = st.qout;
+            st.hubcount = nhub;
+            st.qout = 0;
+            st.hubout = 0;
+            st.visited += nnew;
+            st.frontier_edges = a;
+            st.frontier_sq = q;
+        }
+    }
+    __syncthreads();
+    if (!st.done) {
+        const int32_t c = choose(p, st, g, f, h1, h2, out);
+        __syncthreads();
+        if (tid == 0) st.choice = c;
+    }
+}
+
+// Level 0: the roots (roots[0..nroots), or just `root` when roots is NULL)
+// are the whole frontier; a repeated root counts once.
+__global__ __launch_bounds__(kBlock) void bfs_engine_init(Graph g, Buffers b, PolicyArgs p, uint32_t root,
+                                                          const uint32_t *roots, uint32_t nroots)
+{
+    __shared__ float f[kFeatures], h1[kHidden], h2[kHidden], out[kMaxOut];
+    BfsState &st = *b.st;
+    if (threadIdx.x == 0) {
+        uint32_t nq = 0, nh = 0;
+        uint64_t visited = 0, edges = 0;
+        double sq = 0.0;
+        b.hub_prefix[0] = 0;
+        for (uint32_t i = 0; i < (roots ? nroots : 1u); i++) {
+            const uint32_t r = roots ? roots[i] : root;
+            if (b.levels[r] == 0) continue;
+            const uint32_t dr = out_degree(g, r);
+            b.levels[r] = 0;
+            b.bits[0][r >> 5] |= 1u << (r & 31);
+            b.vis[r >> 5] |= 1u << (r & 31);
+            visited++;
+            edges += dr;
+            sq += static_cast<double>(dr) * dr;
+            if (dr > b.hub_degree && nh < kHubCap) {
+                b.hubq[0][nh] = r;
+                b.hub_prefix[nh + 1] = b.hub_prefix[nh] + dr;
+                nh++;
+            } else {
+                b.q[0][nq++] = r;
+            }
+        }
+        st.depth = 0;
+        st.done = 0;
+        st.qout = 0;
+        st.hubout = 0;
+        st.num_levels = 0;
+        st.visited = visited;
+        st.frontier_edges = edges;
+        st.frontier_sq = sq;
+        st.prev_choice = -1;
+        st.prev_time_ns = 0.0f;
+        st.t_step = 0;
+        st.qcount = nq;
+        st.hubcount = nh;
+        st.choice = p.fixed;
+    }
+    __syncthreads();
+    const int32_t c = choose(p, st, g, f, h1, h2, out);
+    __syncthreads();
+    if (threadIdx.x == 0) st.choice = c;
+}
+
+__global__ void set_choice(BfsState *st, int32_t c) { st->choice = c; }
+__global__ void copy_state(BfsState *dst, const BfsState *src) { *dst = *src; }
+
+}  // namespace
+}  // namespace bw
+
+using namespace bw;
+
+struct bw_bfs_engine {
+    Graph g{};
+    Buffers b{};
+    uint32_t grid = 0;
+    int32_t *own_levels = nullptr;
+    // Snapshot of levels, queues, bitmaps and state, for timing every
+    // variant on the same level.
+    int32_t *snap_levels = nullptr;
+    uint32_t *snap_q = nullptr, *snap_hub = nullptr, *snap_bits = nullptr;
+    uint64_t *snap_prefix = nullptr;
+    BfsState *snap_st = nullptr;
+    float *mlp = nullptr;
+    int32_t mlp_out = 0;
+    int32_t *schedule = nullptr;
+    int32_t schedule_cap = 0;
+    uint32_t *roots = nullptr;
+    uint32_t roots_cap = 0;
+    hipStream_t own_stream = nullptr;
+    hipGraphExec_t exec = nullptr;
+    int32_t exec_key[4] = {-1, -1, -1, -1};  // policy, fixed, schedule_len, levels per batch
+    void *exec_stream = nullptr;
+
+    void drop_exec()
+    {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        exec = nullptr;
+    }
+
+    ~bw_bfs_engine()
+    {
+        drop_exec();
+        void *bufs[] = {own_levels, b.q[0], b.q[1], b.hubq[0], b.hubq[1], b.hub_prefix, b.bits[0], b.bits[1],
+                        b.bits[2], b.vis, b.st, b.part, b.counts, b.times, b.choices, snap_levels, snap_q, snap_hub,
+                        snap_bits, snap_prefix, snap_st, mlp, schedule, roots};
+        for (void *p : bufs)
+            if (p) (void)hipFree(p);
+        if (own_stream) (void)hipStreamDestroy(own_stream);
+    }
+};
+
+namespace {
+
+PolicyArgs make_policy(const bw_bfs_engine *e, int32_t policy, int32_t fixed, int32_t schedule_len)
+{
+    PolicyArgs p;
+    p.policy = policy;
+    p.fixed = fixed;
+    p.schedule = e->schedule;
+    p.schedule_len = std::max(schedule_len, 1);
+    p.mlp = e->mlp;
+    p.nout = e->mlp_out;
+    return p;
+}
+
+int check_variant(int32_t v)
+{
+    if (v < 0 || v >= kVariants) return fail(BW_HIP_ERR_ARG, "unknown native BFS variant " + std::to_string(v));
+    return BW_HIP_OK;
+}
+
+int check_policy(const bw_bfs_engine *e, int32_t policy, int32_t fixed, int32_t schedule_len)
+{
+    BW_TRY(check_variant(fixed));
+    if (policy < FIXED || policy > HOST) return fail(BW_HIP_ERR_ARG, "unknown policy " + std::to_string(policy));
+    if (policy == LEARNED && !e->mlp) return fail(BW_HIP_ERR_ARG, "no model set for the learned policy");
+    if (policy == SCHEDULE && (!e->schedule || schedule_len < 1 || schedule_len > e->schedule_cap))
+        return fail(BW_HIP_ERR_ARG, "no schedule of that length set");
+    return BW_HIP_OK;
+}
+
+int launch_level(bw_bfs_engine *e, const PolicyArgs &p, hipStream_t s)
+{
+    hipLaunchKernelGGL(bfs_engine_step, dim3(e->grid), dim3(kBlock), 0, s, e->g, e->b);
+    BW_LAUNCH_CHECK();
+    hipLaunchKernelGGL(bfs_engine_epilogue, dim3(1), dim3(kBlock), 0, s, e->g, e->b, p, e->grid);
+    BW_LAUNCH_CHECK();
+    return BW_HIP_OK;
+}
+
+}  // namespace
+
+extern "C" int bw_bfs_engine_create(bw_bfs_engine **out, const bw_graph_dev *g, int32_t *levels, uint32_t max_levels)
+{
+    if (!out || !g || max_levels == 0) return fail(BW_HIP_ERR_ARG, "bw_bfs_engine_create: invalid argument");
+    *out = nullptr;
+    if (g->offset_bytes != 4 || !g->offsets || !g->rev_offsets || (g->edge_count && (!g->cols || !g->rev_cols)))
+        return fail(BW_HIP_ERR_ARG, "the BFS engine needs both CSR directions with uint32 offsets");
+    if (g->vertex_count == 0 || g->vertex_count >= 0x7FFFFFFFull)
+        return fail(BW_HIP_ERR_ARG, "the BFS engine needs 0 < V < 2^31");
+    int dev = 0, rate_khz = 0;
+    BW_HIP_CHECK(hipGetDevice(&dev));
+    BW_HIP_CHECK(hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, dev));
+    if (rate_khz <= 0) return fail(BW_HIP_ERR_RUNTIME, "device reports no wall clock rate");
+    auto e = new bw_bfs_engine();
+    const uint64_t V = g->vertex_count;
+    e->g = Graph{V, g->edge_count, static_cast<const uint32_t *>(g->offsets), g->cols,
+                 static_cast<const uint32_t *>(g->rev_offsets), g->rev_cols};
+    e->grid = static_cast<uint32_t>(device_cus()) * 8;
+    const uint32_t words = static_cast<uint32_t>((V + 31) / 32);
+    bool ok = true;
+    auto alloc = [&](auto **p, size_t bytes) {
+        if (ok && hipMalloc(reinterpret_cast<void **>(p), std::max<size_t>(bytes, 8)) != hipSuccess) ok = false;
+    };
+    if (!levels) {
+        alloc(&e->own_levels, sizeof(int32_t) * V);
+        levels = e->own_levels;
+    }
+    e->b.levels = levels;
+    alloc(&e->b.q[0], sizeof(uint32_t) * V);
+    alloc(&e->b.q[1], sizeof(uint32_t) * V);
+    alloc(&e->b.hubq[0], sizeof(uint32_t) * kHubCap);
+    alloc(&e->b.hubq[1], sizeof(uint32_t) * kHubCap);
+    alloc(&e->b.hub_prefix, sizeof(uint64_t) * (kHubCap + 1));
+    for (auto &bm : e->b.bits) alloc(&bm, sizeof(uint32_t) * words);
+    alloc(&e->b.vis, sizeof(uint32_t) * words);
+    alloc(&e->b.st, sizeof(BfsState));
+    alloc(&e->b.part, sizeof(uint64_t) * 2 * e->grid);
+    alloc(&e->b.counts, sizeof(uint32_t) * max_levels);
+    alloc(&e->b.times, sizeof(uint64_t) * max_levels);
+    alloc(&e->b.choices, sizeof(int32_t) * max_levels);
+    e->b.max_levels = max_levels;
+    e->b.words = words;
+    e->b.ns_per_tick = 1e6f / static_cast<float>(rate_khz);
+    e->b.hub_degree = BW_BFS_DEFAULT_HUB_DEGREE;
+    e->b.wave_degree = BW_BFS_DEFAULT_WAVE_DEGREE;
+    if (!ok) {
+        delete e;
+        return fail(BW_HIP_ERR_NOMEM, "device allocation of the BFS engine failed");
+    }
+    *out = e;
+    return BW_HIP_OK;
+}
+
+extern "C" int bw_bfs_engine_destroy(bw_bfs_engine *e)
+{
+    delete e;
+    return BW_HIP_OK;
+}
+
+extern "C" int bw_bfs_engine_levels(const bw_bfs_engine *e, int32_t **levels)
+{
+    if (!e || !levels) return fail(BW_HIP_ERR_ARG, "invalid argument");
+    *levels = e->b.levels;
+    return BW_HIP_OK;
+}
+
+extern "C" int32_t bw_bfs_engine_mlp_len(int32_t outputs) { return mlp_len(outputs); }
+
+extern "C" int bw_bfs_engine_set_model(bw_bfs_engine *e, const float *weights, int32_t outputs)
+{
+    if (!e || !weights || outputs != kVariants)
+        return fail(BW_HIP_ERR_ARG, "bw_bfs_engine_set_model: the model needs one output per native variant");
+    if (!e->mlp) BW_HIP_CHECK(hipMalloc(&e->mlp, sizeof(float) * mlp_len(kMaxOut)));
+    BW_HIP_CHECK(hipMemcpy(e->mlp, weights, sizeof(float) * mlp_len(outputs), hipMemcpyHostToDevice));
+    e->mlp_out = outputs;
+    return BW_HIP_OK;
+}
+
+extern "C" int bw_bfs_engine_set_schedule(bw_bfs_engine *e, const int32_t *schedule, int32_t len)
+{
+    if (!e || !schedule || len < 1) return fail(BW_HIP_ERR_ARG, "invalid schedule");
+    for (int32_t i = 0; i < len; i++) BW_TRY(check_variant(schedule[i]));
+    if (len > e->schedule_cap) {
+        e->drop_exec();  // the captured batch holds the old schedule pointer
+        if (e->schedule) (void)hipFree(e->schedule);
+        e->schedule = nullptr;
+        e->schedule_cap = 0;
+        BW_HIP_CHECK(hipMalloc(&e->schedule, sizeof(int32_t) * len));
+        e->schedule_cap = len;
+    }
+    BW_HIP_CHECK(hipMemcpy(e->schedule, schedule, sizeof(int32_t) * len, hipMemcpyHostToDevice));
+    return BW_HIP_OK;
+}
+
+extern "C" int bw_bfs_engine_set_tuning(bw_bfs_engine *e, uint32_t hub_degree, uint32_t wave_degree)
+{
+    if (!e) return fail(BW_HIP_ERR_ARG, "engine is NULL");
+    if (hub_degree < 256) return fail(BW_HIP_ERR_ARG, "hub_degree must be at least 256");
+    e->drop_exec();  // the captured batch holds the old thresholds
+    e->b.hub_degree = hub_degree;
+    e->b.wave_degree = wave_degree;
+    return BW_HIP_OK;
+}
+
+namespace {
+
+// Resets levels, bitmaps and state; roots_dev (device, n entries) or the
+// single `root` form the depth-0 frontier.
+int begin_impl(bw_bfs_engine *e, uint32_t root, const uint32_t *roots_dev, uint32_t nroots, int32_t policy,
+               int32_t fixed, int32_t schedule_len, hipStream_t s)
+{
+    BW_TRY(check_policy(e, policy, fixed, schedule_len));
+    BW_TRY(bw_hip_fill_u32(e->b.levels, static_cast<uint32_t>(kUnvisited), e->g.V, s));
+    for (uint32_t *bm : e->b.bits) BW_TRY(bw_hip_fill_u32(bm, 0u, e->b.words, s));
+    BW_TRY(bw_hip_fill_u32(e->b.vis, 0u, e->b.words, s));
+    hipLaunchKernelGGL(bfs_engine_init, dim3(1), dim3(kBlock), 0, s, e->g, e->b,
+                       make_policy(e, policy, fixed, schedule_len), root, roots_dev, nroots);
+    BW_LAUNCH_CHECK();
+    return BW_HIP_OK;
+}
+
+// Copies host roots into the engine's device buffer (checked in range).
+int upload_roots(bw_bfs_engine *e, const uint32_t *roots, uint32_t n, hipStream_t s)
+{
+    if (!roots || n == 0 || n > BW_BFS_MAX_ROOTS)
+        return fail(BW_HIP_ERR_ARG, "roots must hold 1.." + std::to_string(BW_BFS_MAX_ROOTS) + " vertices");
+    for (uint32_t i = 0; i < n; i++)
+        if (roots[i] >= e->g.V) return fail(BW_HIP_ERR_ARG, "root " + std::to_string(roots[i]) + " out of range");
+    if (n > e->roots_cap) {
+        if (e->roots) (void)hipFree(e->roots);
+        e->roots = nullptr;
+        e->roots_cap = 0;
+        BW_HIP_CHECK(hipMalloc(&e->roots, sizeof(uint32_t) * n));
+        e->roots_cap = n;
+    }
+    BW_HIP_CHECK(hipMemcpyAsync(e->roots, roots, sizeof(uint32_t) * n, hipMemcpyHostToDevice, s));
+    BW_HIP_CHECK(hipStreamSynchronize(s));
+    return BW_HIP_OK;
+}
+
+}  // namespace
+
+// Resets levels, bitmaps and state for a BFS from `root`; the level-0
+// variant is chosen by the policy.
+extern "C" int bw_bfs_engine_begin(bw_bfs_engine *e, uint32_t root, int32_t policy, int32_t fixed,
+                                   int32_t schedule_len, void *stream)
+{
+    if (!e) return fail(BW_HIP_ERR_ARG, "engine is NULL");
+    if (root >= e->g.V) return fail(BW_HIP_ERR_ARG, "root out of range");
+    return begin_impl(e, root, nullptr, 0, policy, fixed, schedule_len, static_cast<hipStream_t>(stream));
+}
+
+extern "C" int bw_bfs_engine_begin_multi(bw_bfs_engine *e, const uint32_t *roots_host, uint32_t nroots,
+                                         int32_t policy, int32_t fixed, int32_t schedule_len, void *stream)
+{
+    if (!e) return fail(BW_HIP_ERR_ARG, "engine is NULL");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    BW_TRY(upload_roots(e, roots_host, nroots, s));
+    return begin_impl(e, 0, e->roots, nroots, policy, fixed, schedule_len, s);
+}
+
+// One level (step kernel + epilogue) with the given policy choosing the
+// variant of the level after it.
+extern "C" int bw_bfs_engine_level(bw_bfs_engine *e, int32_t policy, int32_t fixed, int32_t schedule_len,
+                                   void *stream)
+{
+    if (!e) return fail(BW_HIP_ERR_ARG, "engine is NULL");
+    BW_TRY(check_policy(e, policy, fixed, schedule_len));
+    return launch_level(e, make_policy(e, policy, fixed, schedule_len), static_cast<hipStream_t>(stream));
+}
+
+// Overrides the variant of the pending level.
+extern "C" int bw_bfs_engine_force(bw_bfs_engine *e, int32_t variant, void *stream)
+{
+    if (!e) return fail(BW_HIP_ERR_ARG, "engine is NULL");
+    BW_TRY(check_variant(variant));
+    hipLaunchKernelGGL(set_choice, dim3(1), dim3(1), 0, static_cast<hipStream_t>(stream), e->b.st, variant);
+    BW_LAUNCH_CHECK();
+    return BW_HIP_OK;
+}
+
+// Host view of the state (synchronises the stream): done flag, depth,
+// levels finished, the pending variant and the 8 features of the pending
+// level.
+extern "C" int bw_bfs_engine_status(bw_bfs_engine *e, uint32_t *done, uint32_t *depth, uint32_t *num_levels,
+                                    int32_t *choice, float *feat, void *stream)
+{
+    if (!e) return fail(BW_HIP_ERR_ARG, "engine is NULL");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    BfsState st;
+    BW_HIP_CHECK(hipMemcpyAsync(&st, e->b.st, sizeof st, hipMemcpyDeviceToHost, s));
+    BW_HIP_CHECK(hipStreamSynchronize(s));
+    if (done) *done = st.done;
+    if (depth) *depth = st.depth;
+    if (num_levels) *num_levels = st.num_levels;
+    if (choice) *choice = st.choice;
+    if (feat) features(st, e->g.V, feat);
+    return BW_HIP_OK;
+}
+
+// restore == 0 saves levels, frontier queues, bitmaps and state; restore
+// != 0 puts the saved copy back.  Synchronises the stream.
+extern "C" int bw_bfs_engine_snapshot(bw_bfs_engine *e, int32_t restore, void *stream)
+{
+    if (!e) return fail(BW_HIP_ERR_ARG, "engine is NULL");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const uint64_t V = e->g.V;
+    const size_t words = e->b.words;
+    if (!e->snap_levels) {
+        if (restore) return fail(BW_HIP_ERR_ARG, "no snapshot to restore");
+        BW_HIP_CHECK(hipMalloc(&e->snap_levels, sizeof(int32_t) * V));
+        BW_HIP_CHECK(hipMalloc(&e->snap_q, sizeof(uint32_t) * V));
+        BW_HIP_CHECK(hipMalloc(&e->snap_hub, sizeof(uint32_t) * kHubCap));
+        BW_HIP_CHECK(hipMalloc(&e->snap_bits, sizeof(uint32_t) * 4 * words));
+        BW_HIP_CHECK(hipMalloc(&e->snap_prefix, sizeof(uint64_t) * (kHubCap + 1)));
+        BW_HIP_CHECK(hipMalloc(&e->snap_st, sizeof(BfsState)));
+    }
+    BfsState st;
+    BW_HIP_CHECK(hipMemcpyAsync(&st, restore ? e->snap_st : e->b.st, sizeof st, hipMemcpyDeviceToHost, s));
+    BW_HIP_CHECK(hipStreamSynchronize(s));
+    const uint32_t cur = st.depth & 1;
+    const size_t nq = std::max<size_t>(st.qcount, 1), nh = std::max<size_t>(std::min(st.hubcount, kHubCap), 1);
+    auto copy = [&](void *saved, void *live, size_t bytes) {
+        return restore ? hipMemcpyAsync(live, saved, bytes, hipMemcpyDeviceToDevice, s)
+                       : hipMemcpyAsync(saved, live, bytes, hipMemcpyDeviceToDevice, s);
+    };
+    BW_HIP_CHECK(copy(e->snap_levels, e->b.levels, sizeof(int32_t) * V));
+    BW_HIP_CHECK(copy(e->snap_q, e->b.q[cur], sizeof(uint32_t) * nq));
+    BW_HIP_CHECK(copy(e->snap_hub, e->b.hubq[cur], sizeof(uint32_t) * nh));
+    BW_HIP_CHECK(copy(e->snap_prefix, e->b.hub_prefix, sizeof(uint64_t) * (kHubCap + 1)));
+    for (int i = 0; i < 3; i++) BW_HIP_CHECK(copy(e->snap_bits + i * words, e->b.bits[i], sizeof(uint32_t) * words));
+    BW_HIP_CHECK(copy(e->snap_bits + 3 * words, e->b.vis, sizeof(uint32_t) * words));
+    if (restore) hipLaunchKernelGGL(copy_state, dim3(1), dim3(1), 0, s, e->b.st, e->snap_st);
+    else hipLaunchKernelGGL(copy_state, dim3(1), dim3(1), 0, s, e->snap_st, e->b.st);
+    BW_LAUNCH_CHECK();
+    BW_HIP_CHECK(hipStreamSynchronize(s));
+    return BW_HIP_OK;
+}
+
+// Host-driven level with a forced variant, timed with events around the
+// step kernel; for exhaustive per-level timing.  Synchronises the stream.
+extern "C" int bw_bfs_engine_step_forced(bw_bfs_engine *e, int32_t variant, float *ms, uint32_t *count,
+                                         uint32_t *done, void *stream)
+{
+    if (!e) return fail(BW_HIP_ERR_ARG, "engine is NULL");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    BfsState before;
+    BW_HIP_CHECK(hipMemcpyAsync(&before, e->b.st, sizeof before, hipMemcpyDeviceToHost, s));
+    BW_HIP_CHECK(hipStreamSynchronize(s));
+    if (before.done) return fail(BW_HIP_ERR_ARG, "the search is finished");
+    BW_TRY(bw_bfs_engine_force(e, variant, stream));
+    hipEvent_t ev0, ev1;
+    BW_HIP_CHECK(hipEventCreate(&ev0));
+    BW_HIP_CHECK(hipEventCreate(&ev1));
+    int st = BW_HIP_OK;
+    if (hipEventRecord(ev0, s) != hipSuccess) st = fail(BW_HIP_ERR_RUNTIME, "event record failed");
+    if (st == BW_HIP_OK) {
+        hipLaunchKernelGGL(bfs_engine_step, dim3(e->grid), dim3(kBlock), 0, s, e->g, e->b);
+        hipError_t le = hipGetLastError();
+        if (le != hipSuccess) st = hip_fail(le, "bfs_engine_step", __FILE__, __LINE__);
+    }
+    if (st == BW_HIP_OK && hipEventRecord(ev1, s) != hipSuccess) st = fail(BW_HIP_ERR_RUNTIME, "event record failed");
+    if (st == BW_HIP_OK) {
+        hipLaunchKernelGGL(bfs_engine_epilogue, dim3(1), dim3(kBlock), 0, s, e->g, e->b,
+                           make_policy(e, HOST, variant, 1), e->grid);
+        hipError_t le = hipGetLastError();
+        if (le != hipSuccess) st = hip_fail(le, "bfs_engine_epilogue", __FILE__, __LINE__);
+    }
+    float t = 0;
+    if (st == BW_HIP_OK && (hipStreamSynchronize(s) != hipSuccess || hipEventElapsedTime(&t, ev0, ev1) != hipSuccess))
+        st = fail(BW_HIP_ERR_RUNTIME, "timing the level failed");
+    (void)hipEventDestroy(ev0);
+    (void)hipEventDestroy(ev1);
+    if (st != BW_HIP_OK) return st;
+    BfsState after;
+    BW_HIP_CHECK(hipMemcpy(&after, e->b.st, sizeof after, hipMemcpyDeviceToHost));
+    if (ms) *ms = t;
+    if (count) BW_HIP_CHECK(hipMemcpy(count, e->b.counts + before.depth, sizeof(uint32_t), hipMemcpyDeviceToHost));
+    if (done) *done = after.done;
+    return BW_HIP_OK;
+}
+
+// The whole search on the device: levels are launched in batches of
+// `levels_per_sync` (replayed as one captured hipGraph when use_graph != 0)
+// and the host reads only the done flag between batches.  Per-level counts,
+// device times and variants are copied out at the end when requested.
+namespace {
+
+int run_impl(bw_bfs_engine *e, uint32_t root, const uint32_t *roots_host, uint32_t nroots, int32_t policy,
+             int32_t fixed, int32_t schedule_len, int32_t levels_per_sync, int32_t use_graph, uint32_t *counts_host,
+             uint64_t *times_ns_host, int32_t *choices_host, uint32_t *num_levels, float *total_ms, void *stream)
+{
+    if (!e || !num_levels || levels_per_sync < 1) return fail(BW_HIP_ERR_ARG, "bw_bfs_engine_run: invalid argument");
+    BW_TRY(check_policy(e, policy, fixed, schedule_len));
+    if (!roots_host && root >= e->g.V) return fail(BW_HIP_ERR_ARG, "root out of range");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if (use_graph && !s) {  // the null stream cannot be captured
+        if (!e->own_stream) BW_HIP_CHECK(hipStreamCreateWithFlags(&e->own_stream, hipStreamNonBlocking));
+        s = e->own_stream;
+    }
+    if (roots_host) BW_TRY(upload_roots(e, roots_host, nroots, s));
+    const PolicyArgs p = make_policy(e, policy, fixed, schedule_len);
+    if (use_graph) {
+        const int32_t key[4] = {policy, fixed, std::max(schedule_len, 1), levels_per_sync};
+        if (e->exec && (std::memcmp(key, e->exec_key, sizeof key) != 0 || e->exec_stream != s)) {
+            (void)hipGraphExecDestroy(e->exec);
+            e->exec = nullptr;
+        }
+        if (!e->exec) {
+            // Kernels only: no memset nodes in the captured batch.
+            hipGraph_t graph = nullptr;
+            BW_HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+            int cst = BW_HIP_OK;
+            for (int32_t i = 0; cst == BW_HIP_OK && i < levels_per_sync; i++) cst = launch_level(e, p, s);
+            hipError_t ce = hipStreamEndCapture(s, &graph);
+            if (cst != BW_HIP_OK) {
+                if (graph) (void)hipGraphDestroy(graph);
+                return cst;
+            }
+            if (ce != hipSuccess) return hip_fail(ce, "hipStreamEndCapture", __FILE__, __LINE__);
+            hipError_t ie = hipGraphInstantiate(&e->exec, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            if (ie != hipSuccess) {
+                e->exec = nullptr;
+                return hip_fail(ie, "hipGraphInstantiate", __FILE__, __LINE__);
+            }
+            std::memcpy(e->exec_key, key, sizeof key);
+            e->exec_stream = s;
+        }
+    }
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    BW_HIP_CHECK(hipEventCreate(&ev0));
+    BW_HIP_CHECK(hipEventCreate(&ev1));
+    int st = BW_HIP_OK;
+    if (hipEventRecord(ev0, s) != hipSuccess) st = fail(BW_HIP_ERR_RUNTIME, "event record failed");
+    if (st == BW_HIP_OK)
+        st = begin_impl(e, root, roots_host ? e->roots : nullptr, nroots, policy, fixed, schedule_len, s);
+    uint32_t done = 0;
+    uint32_t batches = 0;
+    const uint32_t max_batches = e->b.max_levels / static_cast<uint32_t>(levels_per_sync) + 2;
+    while (st == BW_HIP_OK && !done) {
+        if (batches++ > max_batches) {
+            st = fail(BW_HIP_ERR_LIMIT, "BFS engine exceeded its level bound");
+            break;
+        }
+        if (use_graph) {
+            hipError_t le = hipGraphLaunch(e->exec, s);
+            if (le != hipSuccess) st = hip_fail(le, "hipGraphLaunch", __FILE__, __LINE__);
+        } else {
+            for (int32_t i = 0; st == BW_HIP_OK && i < levels_per_sync; i++) st = launch_level(e, p, s);
+        }
+        if (st != BW_HIP_OK) break;
+        if (hipMemcpyAsync(&done, &e->b.st->done, sizeof done, hipMemcpyDeviceToHost, s) != hipSuccess ||
+            hipStreamSynchronize(s) != hipSuccess)
+            st = fail(BW_HIP_ERR_RUNTIME, "reading the BFS state failed");
+    }
+    if (st == BW_HIP_OK) {
+        if (hipEventRecord(ev1, s) != hipSuccess || hipEventSynchronize(ev1) != hipSuccess)
+            st = fail(BW_HIP_ERR_RUNTIME, "event failed");
+        else if (total_ms)
+            (void)hipEventElapsedTime(total_ms, ev0, ev1);
+    }
+    (void)hipEventDestroy(ev0);
+    (void)hipEventDestroy(ev1);
+    if (st != BW_HIP_OK) return st;
+    BfsState hs;
+    BW_HIP_CHECK(hipMemcpy(&hs, e->b.st, sizeof hs, hipMemcpyDeviceToHost));
+    *num_levels = hs.num_levels;
+    if (hs.num_levels >= e->b.max_levels) {
+        uint32_t last = 0;
+        BW_HIP_CHECK(hipMemcpy(&last, e->b.counts + e->b.max_levels - 1, sizeof last, hipMemcpyDeviceToHost));
+        if (last != 0)
+            return fail(BW_HIP_ERR_LIMIT, "BFS reached max_levels = " + std::to_string(e->b.max_levels) +
+                                              " with a non-empty frontier");
+    }
+    const size_t n = std::min<uint32_t>(hs.num_levels, e->b.max_levels);
+    if (counts_host && n) BW_HIP_CHECK(hipMemcpy(counts_host, e->b.counts, sizeof(uint32_t) * n, hipMemcpyDeviceToHost));
+    if (times_ns_host && n)
+        BW_HIP_CHECK(hipMemcpy(times_ns_host, e->b.times, sizeof(uint64_t) * n, hipMemcpyDeviceToHost));
+    if (choices_host && n)
+        BW_HIP_CHECK(hipMemcpy(choices_host, e->b.choices, sizeof(int32_t) * n, hipMemcpyDeviceToHost));
+    return BW_HIP_OK;
+}
+
+}  // namespace
+
+extern "C" int bw_bfs_engine_run(bw_bfs_engine *e, uint32_t root, int32_t policy, int32_t fixed,
+                                 int32_t schedule_len, int32_t levels_per_sync, int32_t use_graph,
+                                 uint32_t *counts_host, uint64_t *times_ns_host, int32_t *choices_host,
+                                 uint32_t *num_levels, float *total_ms, void *stream)
+{
+    return run_impl(e, root, nullptr, 0, policy, fixed, schedule_len, levels_per_sync, use_graph, counts_host,
+                    times_ns_host, choices_host, num_levels, total_ms, stream);
+}
+
+extern "C" int bw_bfs_engine_run_multi(bw_bfs_engine *e, const uint32_t *roots_host, uint32_t nroots,
+                                       int32_t policy, int32_t fixed, int32_t schedule_len, int32_t levels_per_sync,
+                                       int32_t use_graph, uint32_t *counts_host, uint64_t *times_ns_host,
+                                       int32_t *choices_host, uint32_t *num_levels, float *total_ms, void *stream)
+{
+    if (!roots_host) return fail(BW_HIP_ERR_ARG, "roots is NULL");
+    return run_impl(e, 0, roots_host, nroots, policy, fixed, schedule_len, levels_per_sync, use_graph, counts_host,
+                    times_ns_host, choices_host, num_levels, total_ms, stream);
+}

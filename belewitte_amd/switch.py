@@ -1,0 +1,265 @@
+"""The learned per-level BFS switch: data, on-GPU training, evaluation.
+
+The reference switches BFS implementations per level through a model
+looked up in SwitchImplementation (ImplementationTemplate.hpp:545-908)
+and judges it against the best non-switching implementation and the
+per-level optimum (benchmark-analysis/src/StepAggregate.hs:95-119,
+benchmark-analysis/model-src/Evaluate.hs:229-230).  Here:
+
+* :func:`collect` times every native variant at every level from the same
+  saved state (exhaustive per-level timing) and records the level's
+  features;
+* :class:`Model` is the FP32 MLP 8 -> 32 -> 32 -> K, trained by
+  :meth:`Model.train` with the hand-written GPU kernels (bw_mlp_train),
+  predicting the log2 level time of each variant;
+* :func:`evaluate` compares, per source, the per-level oracle (sum over
+  levels of the fastest variant), the best fixed variant (the variant with
+  the least total time) and the learned switch, both simulated on the
+  exhaustive timings and executed end to end by the engine with the model
+  evaluated on the device.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import json
+from dataclasses import dataclass
+from pathlib import Path
+
+import numpy as np
+
+from . import _native as N
+from .bfs_native import NUM_FEATURES, VARIANTS, BFSEngine, LevelRecord, teps_edges
+from .device import DeviceArray
+
+HIDDEN = 32
+
+
+def num_params(outputs: int = len(VARIANTS)) -> int:
+    return int(N.hip_lib().bw_mlp_params(outputs))
+
+
+def _layout(K: int) -> dict:
+    F, H = NUM_FEATURES, HIDDEN
+    o = {"mean": 0, "std": F, "W1": 2 * F}
+    o["b1"] = o["W1"] + H * F
+    o["W2"] = o["b1"] + H
+    o["b2"] = o["W2"] + H * H
+    o["W3"] = o["b2"] + H
+    o["b3"] = o["W3"] + K * H
+    o["end"] = o["b3"] + K
+    return o
+
+
+@dataclass
+class Samples:
+    X: np.ndarray       # float32[n][8] features
+    ms: np.ndarray      # float64[n][K] level time of each variant
+    source: np.ndarray  # int64[n]
+    depth: np.ndarray   # int64[n]
+
+    @classmethod
+    def from_records(cls, records) -> "Samples":
+        recs = list(records)
+        return cls(np.array([r.features for r in recs], dtype=np.float32).reshape(-1, NUM_FEATURES),
+                   np.array([r.ms for r in recs], dtype=np.float64).reshape(len(recs), -1),
+                   np.array([r.source for r in recs], dtype=np.int64),
+                   np.array([r.depth for r in recs], dtype=np.int64))
+
+    @property
+    def targets(self) -> np.ndarray:
+        """log2 of the level time in ns (the regression target)."""
+        return np.log2(np.maximum(self.ms, 1e-6) * 1e6).astype(np.float32)
+
+    def concat(self, other: "Samples") -> "Samples":
+        return Samples(np.concatenate([self.X, other.X]), np.concatenate([self.ms, other.ms]),
+                       np.concatenate([self.source, other.source]), np.concatenate([self.depth, other.depth]))
+
+    def save(self, path) -> None:
+        np.savez(path, X=self.X, ms=self.ms, source=self.source, depth=self.depth)
+
+    @classmethod
+    def load(cls, path) -> "Samples":
+        with np.load(path, allow_pickle=False) as z:
+            return cls(z["X"], z["ms"], z["source"], z["depth"])
+
+
+class Model:
+    """Packed FP32 weights of the MLP (mean, std, W1, b1, W2, b2, W3, b3)."""
+
+    def __init__(self, weights: np.ndarray, outputs: int = len(VARIANTS)):
+        self.K = int(outputs)
+        self.w = np.ascontiguousarray(weights, dtype=np.float32)
+        if self.w.size != num_params(self.K):
+            raise ValueError(f"expected {num_params(self.K)} parameters, got {self.w.size}")
+
+    @classmethod
+    def init(cls, X: np.ndarray, outputs: int = len(VARIANTS), seed: int = 0, target_mean: float = 0.0):
+        """He-initialised weights; mean/std taken from the training features."""
+        K = outputs
+        L = _layout(K)
+        rng = np.random.default_rng(seed)
+        w = np.zeros(L["end"], dtype=np.float32)
+        X = np.asarray(X, dtype=np.float64)
+        w[L["mean"]:L["std"]] = X.mean(axis=0) if X.size else 0.0
+        sd = X.std(axis=0) if X.size else np.ones(NUM_FEATURES)
+        w[L["std"]:L["W1"]] = np.where(sd > 1e-6, sd, 1.0)
+        w[L["W1"]:L["b1"]] = rng.normal(0, np.sqrt(2.0 / NUM_FEATURES), HIDDEN * NUM_FEATURES)
+        w[L["W2"]:L["b2"]] = rng.normal(0, np.sqrt(2.0 / HIDDEN), HIDDEN * HIDDEN)
+        w[L["W3"]:L["b3"]] = rng.normal(0, np.sqrt(1.0 / HIDDEN), K * HIDDEN)
+        w[L["b3"]:L["end"]] = target_mean
+        return cls(w, K)
+
+    # ------------------------------------------------------------ device
+
+    def predict(self, X: np.ndarray) -> np.ndarray:
+        """Predicted log2 level times, float32[n][K] (bw_mlp_forward)."""
+        X = np.ascontiguousarray(X, dtype=np.float32).reshape(-1, NUM_FEATURES)
+        n = X.shape[0]
+        if n == 0:
+            return np.zeros((0, self.K), dtype=np.float32)
+        dw, dx = DeviceArray.from_host(self.w), DeviceArray.from_host(X)
+        dout = DeviceArray(n * self.K, np.float32)
+        N.check_hip(N.hip_lib().bw_mlp_forward(dw.ptr, self.K, dx.ptr, n, dout.ptr, None))
+        out = dout.download().reshape(n, self.K)
+        for a in (dw, dx, dout):
+            a.free()
+        return out
+
+    def choose(self, X: np.ndarray) -> np.ndarray:
+        return np.argmin(self.predict(X), axis=1)
+
+    def gradient(self, X: np.ndarray, Y: np.ndarray):
+        """(grad, loss) of the mean squared error over <= 128 samples
+        (bw_mlp_grad)."""
+        X = np.ascontiguousarray(X, dtype=np.float32).reshape(-1, NUM_FEATURES)
+        Y = np.ascontiguousarray(Y, dtype=np.float32).reshape(X.shape[0], self.K)
+        dw, dx, dy = DeviceArray.from_host(self.w), DeviceArray.from_host(X), DeviceArray.from_host(Y)
+        dg, dl = DeviceArray(self.w.size, np.float32), DeviceArray(1, np.float32)
+        N.check_hip(N.hip_lib().bw_mlp_grad(dw.ptr, self.K, dx.ptr, dy.ptr, X.shape[0], dg.ptr, dl.ptr, None))
+        g, loss = dg.download(), float(dl.download()[0])
+        for a in (dw, dx, dy, dg, dl):
+            a.free()
+        return g, loss
+
+    def train(self, X: np.ndarray, Y: np.ndarray, epochs: int = 2000, batch: int = 64, lr: float = 3e-3,
+              seed: int = 1) -> np.ndarray:
+        """Trains in place on the GPU (bw_mlp_train); returns the per-epoch loss."""
+        X = np.ascontiguousarray(X, dtype=np.float32).reshape(-1, NUM_FEATURES)
+        Y = np.ascontiguousarray(Y, dtype=np.float32).reshape(X.shape[0], self.K)
+        dw, dx, dy = DeviceArray.from_host(self.w), DeviceArray.from_host(X), DeviceArray.from_host(Y)
+        dl = DeviceArray(max(epochs, 1), np.float32)
+        N.check_hip(N.hip_lib().bw_mlp_train(dw.ptr, self.K, dx.ptr, dy.ptr, X.shape[0], epochs, batch,
+                                             lr, seed, dl.ptr, None))
+        self.w = dw.download()
+        loss = dl.download()[:epochs]
+        for a in (dw, dx, dy, dl):
+            a.free()
+        return loss
+
+    # -------------------------------------------------------------- files
+
+    def save(self, path) -> None:
+        Path(path).write_text(json.dumps({"format": "belewitte-bfs-mlp", "outputs": self.K,
+                                          "variants": list(VARIANTS[:self.K]), "hidden": HIDDEN,
+                                          "features": NUM_FEATURES, "weights": self.w.tolist()}))
+
+    @classmethod
+    def load(cls, path) -> "Model":
+        d = json.loads(Path(path).read_text())
+        if d.get("format") != "belewitte-bfs-mlp":
+            raise ValueError(f"{path} is not a BFS switch model")
+        return cls(np.asarray(d["weights"], dtype=np.float32), int(d["outputs"]))
+
+
+def collect(engine: BFSEngine, sources, repeats: int = 1):
+    """Exhaustive per-level timings of every variant from each source.
+    Returns (Samples, {source: (levels, counts)})."""
+    records, results = [], {}
+    for s in sources:
+        levels, counts, recs = engine.exhaustive(int(s), repeats=repeats)
+        records.extend(recs)
+        results[int(s)] = (levels, counts)
+    return Samples.from_records(records), results
+
+
+def fit(samples: Samples, epochs: int = 2000, batch: int = 64, lr: float = 3e-3, seed: int = 0):
+    """A model trained on the GPU on `samples`; returns (model, epoch losses)."""
+    Y = samples.targets
+    m = Model.init(samples.X, seed=seed, target_mean=float(Y.mean()) if Y.size else 0.0)
+    loss = m.train(samples.X, Y, epochs=epochs, batch=batch, lr=lr, seed=seed + 1)
+    return m, loss
+
+
+def simulate(samples: Samples, choices: np.ndarray) -> dict:
+    """Per-source totals of the per-level oracle, each fixed variant and a
+    given per-level choice, on the exhaustive timings (ms)."""
+    out = {}
+    for s in np.unique(samples.source):
+        m = samples.source == s
+        t = samples.ms[m]
+        out[int(s)] = {"oracle": float(t.min(axis=1).sum()),
+                       "fixed": t.sum(axis=0).tolist(),
+                       "switch": float(t[np.arange(t.shape[0]), choices[m]].sum())}
+    return out
+
+
+def evaluate(engine: BFSEngine, g, sources, model: Model, repeats: int = 1, runs: int = 3,
+             levels_per_sync: int = 8) -> dict:
+    """Learned switch vs best fixed vs per-level oracle on `sources`.
+
+    Simulated: sums of exhaustive per-level timings (device time of each
+    level kernel).  Executed: whole searches run by the engine with the
+    level loop on the device (hipGraph batches), min over `runs`; the
+    learned policy evaluates the model on the device at every level."""
+    samples, results = collect(engine, sources, repeats=repeats)
+    learned_choice = model.choose(samples.X)
+    sim = simulate(samples, learned_choice)
+    K = len(VARIANTS)
+    fixed_tot = np.zeros(K)
+    oracle_tot = switch_tot = 0.0
+    for s in sim.values():
+        fixed_tot += np.asarray(s["fixed"])
+        oracle_tot += s["oracle"]
+        switch_tot += s["switch"]
+    best_fixed = int(np.argmin(fixed_tot))
+
+    engine.set_model(model.w)
+    exec_fixed = np.zeros(K)
+    exec_learned = 0.0
+    edges = 0
+    mismatches = 0
+    for s in sources:
+        levels_ref, counts_ref = results[int(s)]
+        edges += teps_edges(g, levels_ref)
+        for k in range(K):
+            best = np.inf
+            for _ in range(runs):
+                r = engine.run(int(s), "fixed", k, levels_per_sync=levels_per_sync, download=False)
+                best = min(best, r.total_ms)
+            exec_fixed[k] += best
+        best = np.inf
+        for _ in range(runs):
+            r = engine.run(int(s), "learned", best_fixed, levels_per_sync=levels_per_sync, download=False)
+            best = min(best, r.total_ms)
+        exec_learned += best
+        lv = engine.levels()
+        if not (np.array_equal(lv, levels_ref) and np.array_equal(r.counts, counts_ref)):
+            mismatches += 1
+    exec_best = int(np.argmin(exec_fixed))
+    gteps = lambda ms: edges / (ms * 1e-3) / 1e9 if ms > 0 else 0.0  # noqa: E731
+    return {
+        "sources": [int(s) for s in sources],
+        "levels": int(samples.X.shape[0]),
+        "teps_edges": int(edges),
+        "simulated_ms": {"oracle": oracle_tot, "best_fixed": float(fixed_tot[best_fixed]),
+                         "best_fixed_variant": VARIANTS[best_fixed], "learned": switch_tot,
+                         "fixed": dict(zip(VARIANTS, fixed_tot.tolist()))},
+        "simulated_speedup_vs_best_fixed": float(fixed_tot[best_fixed] / switch_tot) if switch_tot else None,
+        "simulated_regret_vs_oracle": float(switch_tot / oracle_tot - 1.0) if oracle_tot else None,
+        "executed_ms": {"best_fixed": float(exec_fixed[exec_best]), "best_fixed_variant": VARIANTS[exec_best],
+                        "learned": float(exec_learned), "fixed": dict(zip(VARIANTS, exec_fixed.tolist()))},
+        "executed_gteps": {"learned": gteps(exec_learned), "best_fixed": gteps(exec_fixed[exec_best])},
+        "executed_speedup_vs_best_fixed": float(exec_fixed[exec_best] / exec_learned) if exec_learned else None,
+        "depth_mismatches": mismatches,
+        "learned_choice_histogram": np.bincount(learned_choice, minlength=K).tolist(),
+    }

@@ -399,6 +399,146 @@ int bw_pr_fused_run(const bw_graph_dev *g, const bw_pr_plan *plan,
                     double *resid_host, float *kernel_ms,
                     const bw_launch *cfg);
 
+/* ------------------------------------------- device-driven BFS (MI355X) */
+
+/* A BFS engine computing the levels of bfs/bfs.cpp:43-118 (depth INT_MAX =
+ * unreached, per-level count = newly discovered vertices) with the level
+ * loop on the device.  Each level runs one of the native variants below,
+ * chosen per level by a policy:
+ *   FIXED     always `fixed`;
+ *   SCHEDULE  schedule[depth % len] (bw_bfs_engine_set_schedule);
+ *   LEARNED   an 8 -> 32 -> 32 -> K MLP evaluated on the device on the 8
+ *             frontier features of the level (bw_bfs_engine_set_model);
+ *             the role of SwitchImplementation's model lookup,
+ *             ImplementationTemplate.hpp:545-908;
+ *   HOST      the variant set by bw_bfs_engine_force.
+ * Needs both CSR directions with uint32 offsets (an undirected graph may
+ * pass the same arrays twice) and V < 2^31. */
+typedef struct bw_bfs_engine bw_bfs_engine;
+
+#define BW_BFS_NATIVE_TD 0      /* top-down, queue + edge-balanced LDS scan */
+#define BW_BFS_NATIVE_BU 1      /* bottom-up, thread per unvisited vertex   */
+#define BW_BFS_NATIVE_BU_WAVE 2 /* bottom-up, wave64 probes for big rows    */
+#define BW_BFS_NATIVE_VARIANTS 3
+
+#define BW_POLICY_FIXED 0
+#define BW_POLICY_SCHEDULE 1
+#define BW_POLICY_LEARNED 2
+#define BW_POLICY_HOST 3
+
+/* Features of a level, reduced on the device: log2(1 + x) of frontier
+ * vertices, frontier out-edges and unvisited vertices; depth; log2(1 + x)
+ * of the mean and standard deviation of the frontier out-degree; previous
+ * variant (-1 at level 0); log2(1 + previous level time in ns). */
+#define BW_BFS_FEATURES 8
+
+/* levels: device int32[V] receiving the depths, or NULL for an
+ * engine-owned array (bw_bfs_engine_levels).  max_levels bounds the depth
+ * (no fixed 1000-level limit). */
+int bw_bfs_engine_create(bw_bfs_engine **engine, const bw_graph_dev *g,
+                         int32_t *levels, uint32_t max_levels);
+int bw_bfs_engine_destroy(bw_bfs_engine *engine);
+int bw_bfs_engine_levels(const bw_bfs_engine *engine, int32_t **levels);
+
+/* Work-division thresholds (the autotuner's knobs): out-degree above which
+ * a discovered vertex is a hub whose edges the next top-down level slices
+ * over all workgroups (>= 256), and in-degree above which BU_WAVE probes
+ * with a whole wave instead of one thread. */
+#define BW_BFS_DEFAULT_HUB_DEGREE 4096
+#define BW_BFS_DEFAULT_WAVE_DEGREE 32
+int bw_bfs_engine_set_tuning(bw_bfs_engine *engine, uint32_t hub_degree,
+                             uint32_t wave_degree);
+
+/* Most roots of one multi-root search. */
+#define BW_BFS_MAX_ROOTS 16384
+
+/* Float count of a packed model with `outputs` outputs: mean[8] std[8]
+ * W1[32][8] b1[32] W2[32][32] b2[32] W3[K][32] b3[K] (row-major, ReLU
+ * hidden layers, outputs = predicted log2 level time per variant; the
+ * argmin is taken). */
+int32_t bw_bfs_engine_mlp_len(int32_t outputs);
+int bw_bfs_engine_set_model(bw_bfs_engine *engine, const float *weights_host,
+                            int32_t outputs);
+int bw_bfs_engine_set_schedule(bw_bfs_engine *engine,
+                               const int32_t *schedule_host, int32_t len);
+
+/* Resets levels and state for a search from `root` and picks the level-0
+ * variant. */
+int bw_bfs_engine_begin(bw_bfs_engine *engine, uint32_t root, int32_t policy,
+                        int32_t fixed, int32_t schedule_len, void *stream);
+/* The same from several roots (host array, all at depth 0; a repeated root
+ * counts once).  On a disjoint union of graphs with one root each, every
+ * component gets exactly its own BFS depths. */
+int bw_bfs_engine_begin_multi(bw_bfs_engine *engine, const uint32_t *roots,
+                              uint32_t nroots, int32_t policy, int32_t fixed,
+                              int32_t schedule_len, void *stream);
+/* Enqueues one level (a no-op on the device once the search is done). */
+int bw_bfs_engine_level(bw_bfs_engine *engine, int32_t policy, int32_t fixed,
+                        int32_t schedule_len, void *stream);
+/* Sets the variant of the pending level. */
+int bw_bfs_engine_force(bw_bfs_engine *engine, int32_t variant, void *stream);
+/* Synchronises `stream` and reports the state and the pending level's
+ * features (feat: float[BW_BFS_FEATURES], optional). */
+int bw_bfs_engine_status(bw_bfs_engine *engine, uint32_t *done,
+                         uint32_t *depth, uint32_t *num_levels,
+                         int32_t *choice, float *feat, void *stream);
+/* restore == 0 saves the search state, != 0 restores it (for timing every
+ * variant on the same level). */
+int bw_bfs_engine_snapshot(bw_bfs_engine *engine, int32_t restore,
+                           void *stream);
+/* Runs the pending level with `variant`; ms = device time of its level
+ * kernel (HIP events), count = vertices it discovered. */
+int bw_bfs_engine_step_forced(bw_bfs_engine *engine, int32_t variant,
+                              float *ms, uint32_t *count, uint32_t *done,
+                              void *stream);
+/* A whole search: levels enqueued in batches of levels_per_sync (replayed
+ * as one captured hipGraph if use_graph; a NULL stream is then replaced by
+ * an engine-owned one), the host reading only the done flag between
+ * batches.  Optional host outputs, max_levels entries each: per-level
+ * counts, device times (ns) and variants.  total_ms = device time from the
+ * reset to the end. */
+int bw_bfs_engine_run(bw_bfs_engine *engine, uint32_t root, int32_t policy,
+                      int32_t fixed, int32_t schedule_len,
+                      int32_t levels_per_sync, int32_t use_graph,
+                      uint32_t *counts_host, uint64_t *times_ns_host,
+                      int32_t *choices_host, uint32_t *num_levels,
+                      float *total_ms, void *stream);
+/* bw_bfs_engine_run from several roots (see bw_bfs_engine_begin_multi). */
+int bw_bfs_engine_run_multi(bw_bfs_engine *engine, const uint32_t *roots,
+                            uint32_t nroots, int32_t policy, int32_t fixed,
+                            int32_t schedule_len, int32_t levels_per_sync,
+                            int32_t use_graph, uint32_t *counts_host,
+                            uint64_t *times_ns_host, int32_t *choices_host,
+                            uint32_t *num_levels, float *total_ms,
+                            void *stream);
+
+/* ------------------------------------------ the variant predictor (MLP) */
+
+/* FP32 MLP 8 -> 32 -> 32 -> K in the packed layout of
+ * bw_bfs_engine_mlp_len, trained and evaluated on the GPU (the role of the
+ * model lookup of SwitchImplementation, ImplementationTemplate.hpp:545-908;
+ * the reference trains its models offline, benchmark-analysis/model-src).
+ * All pointers are device memory.  X is float[n][8] (raw features),
+ * Y float[n][K] (targets: log2 level time per variant). */
+#define BW_MLP_MAX_BATCH 128
+
+int32_t bw_mlp_params(int32_t outputs);
+/* out[n][K] = network(X). */
+int bw_mlp_forward(const float *weights, int32_t outputs, const float *X,
+                   uint64_t n, float *out, void *stream);
+/* Gradient of L = mean over samples and outputs of (out - y)^2 for
+ * 1 <= n <= BW_MLP_MAX_BATCH samples: grad[bw_mlp_params(K)] (zero for the
+ * mean/std entries), loss[1]. */
+int bw_mlp_grad(const float *weights, int32_t outputs, const float *X,
+                const float *Y, uint32_t n, float *grad, float *loss,
+                void *stream);
+/* Minibatch Adam (beta 0.9 / 0.999) for `epochs` passes over the n
+ * samples, in place on `weights` (mean/std untouched), in one launch;
+ * epoch_loss (optional, float[epochs]) receives each epoch's mean loss. */
+int bw_mlp_train(float *weights, int32_t outputs, const float *X,
+                 const float *Y, uint32_t n, uint32_t epochs, uint32_t batch,
+                 float lr, uint32_t seed, float *epoch_loss, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

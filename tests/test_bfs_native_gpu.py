@@ -1,0 +1,261 @@
+"""The device-driven BFS engine, bit-exact against the oracle.
+
+Depths must equal the oracle's textbook queue BFS and the per-level counts
+the oracle's restatement of the reference level loop (bfs/bfs.cpp:93-107:
+newly discovered vertices per level, last 0), for every native variant,
+fixed and per-level schedules, host-stepped and hipGraph-batched loops,
+the learned policy with any model, hubs beyond the hub-queue capacity,
+unreachable vertices and high-diameter graphs.  The frontier features
+reduced on the device are checked against the same quantities computed on
+the host from the oracle's depths.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from belewitte_amd import _native as N
+from belewitte_amd import bfs_native, device, graph, switch
+from oracle import binding as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _many_hubs():
+    """Root -> 4200 hubs, each with 4100 out-edges: more hubs than the hub
+    queue holds (4096), so the rest go through the normal queue."""
+    rng = np.random.default_rng(3)
+    nh, deg, pool = 4200, 4100, 60000
+    V = 1 + nh + pool
+    src = [np.zeros(nh, dtype=np.uint32)]
+    dst = [np.arange(1, nh + 1, dtype=np.uint32)]
+    for h in range(1, nh + 1):
+        t = rng.choice(pool, deg, replace=False).astype(np.uint32) + np.uint32(1 + nh)
+        src.append(np.full(deg, h, dtype=np.uint32))
+        dst.append(t)
+    return graph.from_edges(V, np.concatenate(src), np.concatenate(dst))
+
+
+def _graphs():
+    yield graph.rmat(12, 16, seed=21)
+    yield graph.rmat(11, 8, seed=22, undirected=True)
+    yield graph.erdos_renyi(5000, 20000, seed=7, undirected=True)
+    yield graph.grid2d(64, 48, rewire=0.01, seed=9)
+    yield graph.chain(2000)
+    yield graph.star(9000)
+    yield graph.star(9000, undirected=True)
+    yield graph.mesh(20, 30)
+
+
+@pytest.fixture(scope="module")
+def cases(gpu):
+    out = []
+    for g in _graphs():
+        csr = O.CSR.from_graph(g)
+        dg = device.DeviceGraph(g)
+        eng = bfs_native.BFSEngine(dg)
+        roots = sorted({g.max_out_degree_vertex(), 0, g.vertex_count // 2, g.vertex_count - 1})
+        out.append((g, csr, dg, eng, roots))
+    yield out
+    for _, _, dg, eng, _ in out:
+        eng.close()
+        dg.free()
+
+
+def _want(csr, root):
+    levels, nlev = O.bfs_queue(csr, root)
+    _, counts = O.bfs_run(csr, root, [4])
+    assert counts.size == nlev
+    return levels, counts
+
+
+@pytest.mark.parametrize("variant", bfs_native.VARIANTS)
+@pytest.mark.parametrize("use_graph", [False, True])
+def test_fixed_variant_bit_exact(cases, variant, use_graph):
+    for g, csr, dg, eng, roots in cases:
+        for root in roots:
+            print(f"case {g.name} root={root} variant={variant} graph={use_graph}", flush=True)
+            r = eng.run(root, "fixed", variant, use_graph=use_graph, levels_per_sync=4 if use_graph else 1)
+            want, counts = _want(csr, root)
+            assert np.array_equal(r.levels, want), (g.name, root, variant)
+            assert np.array_equal(r.counts, counts), (g.name, root, variant)
+            assert np.all(r.choices == bfs_native.variant_id(variant))
+
+
+@pytest.mark.parametrize("schedule", [["td", "bu"], ["bu-wave", "td", "bu"], ["bu", "bu", "td", "bu-wave"]])
+def test_schedules_bit_exact(cases, schedule):
+    for g, csr, dg, eng, roots in cases:
+        root = roots[0]
+        r = eng.run(root, "schedule", schedule=schedule, levels_per_sync=8)
+        want, counts = _want(csr, root)
+        assert np.array_equal(r.levels, want), (g.name, schedule)
+        assert np.array_equal(r.counts, counts), (g.name, schedule)
+        ids = [bfs_native.variant_id(v) for v in schedule]
+        assert r.choices.tolist() == [ids[d % len(ids)] for d in range(r.num_levels)]
+
+
+def test_random_model_never_changes_depths(cases):
+    for seed in range(3):
+        m = switch.Model.init(np.random.default_rng(seed).normal(8, 3, (64, 8)), seed=seed, target_mean=10.0)
+        for g, csr, dg, eng, roots in cases:
+            eng.set_model(m.w)
+            for root in roots[:2]:
+                r = eng.run(root, "learned", "td", levels_per_sync=8)
+                want, counts = _want(csr, root)
+                assert np.array_equal(r.levels, want), (g.name, seed)
+                assert np.array_equal(r.counts, counts), (g.name, seed)
+
+
+def test_many_hubs(gpu):
+    g = _many_hubs()
+    csr = O.CSR.from_graph(g)
+    dg = device.DeviceGraph(g)
+    eng = bfs_native.BFSEngine(dg)
+    try:
+        want, counts = _want(csr, 0)
+        for v in bfs_native.VARIANTS:
+            r = eng.run(0, "fixed", v)
+            assert np.array_equal(r.levels, want), v
+            assert np.array_equal(r.counts, counts), v
+    finally:
+        eng.close()
+        dg.free()
+
+
+def test_device_features_match_host(cases):
+    lib = N.hip_lib()
+    for g, csr, dg, eng, roots in cases[:4]:
+        root = roots[0]
+        levels, _ = O.bfs_queue(csr, root)
+        deg = np.diff(g.offsets()).astype(np.float64)
+        V = g.vertex_count
+        N.check_hip(lib.bw_bfs_engine_begin(eng._e, root, bfs_native.POLICIES["host"], 0, 1, None))
+        d = 0
+        while True:
+            st = eng.status()
+            if st["done"]:
+                break
+            assert st["depth"] == d
+            f = st["features"]
+            fr = deg[levels == d]
+            n = fr.size
+            mean = fr.mean() if n else 0.0
+            std = np.sqrt(max((fr ** 2).mean() - mean * mean, 0.0)) if n else 0.0
+            want = [np.log2(1 + n), np.log2(1 + fr.sum()), np.log2(1 + V - np.count_nonzero(levels <= d)),
+                    d, np.log2(1 + mean), np.log2(1 + std)]
+            assert np.allclose(f[:6], want, rtol=1e-5, atol=1e-5), (g.name, d, f, want)
+            assert f[6] == (-1 if d == 0 else (d - 1) % 3)
+            assert (f[7] == 0) == (d == 0)
+            N.check_hip(lib.bw_bfs_engine_force(eng._e, d % 3, None))
+            N.check_hip(lib.bw_bfs_engine_level(eng._e, bfs_native.POLICIES["host"], 0, 1, None))
+            d += 1
+
+
+def test_device_choice_matches_host_model(cases):
+    """The MLP evaluated inside the level loop picks what the inference
+    kernel picks on the same features."""
+    lib = N.hip_lib()
+    X = np.random.default_rng(1).normal(10, 4, (256, 8)).astype(np.float32)
+    m = switch.Model.init(X, seed=5, target_mean=12.0)
+    for g, csr, dg, eng, roots in cases[:4]:
+        eng.set_model(m.w)
+        N.check_hip(lib.bw_bfs_engine_begin(eng._e, roots[0], bfs_native.POLICIES["learned"], 0, 1, None))
+        while True:
+            st = eng.status()
+            if st["done"]:
+                break
+            pred = m.predict(st["features"][None, :])[0]
+            assert st["choice"] == int(np.argmin(pred)), (g.name, st["depth"], pred)
+            N.check_hip(lib.bw_bfs_engine_level(eng._e, bfs_native.POLICIES["learned"], 0, 1, None))
+
+
+def test_exhaustive_records(cases):
+    for g, csr, dg, eng, roots in cases[:5]:
+        root = roots[-1]
+        levels, counts, recs = eng.exhaustive(root, repeats=1)
+        want, wcounts = _want(csr, root)
+        assert np.array_equal(levels, want)
+        assert np.array_equal(counts, wcounts)
+        assert len(recs) == counts.size
+        assert [r.depth for r in recs] == list(range(len(recs)))
+        for r in recs:
+            assert r.ms.shape == (len(bfs_native.VARIANTS),) and np.all(r.ms > 0)
+
+
+def _union(graphs):
+    """Disjoint union of `graphs` (vertex ids shifted) and the shifts."""
+    src, dst, base = [], [], [0]
+    for g in graphs:
+        s, d = g.edges()
+        src.append(s.astype(np.uint64) + base[-1])
+        dst.append(d.astype(np.uint64) + base[-1])
+        base.append(base[-1] + g.vertex_count)
+    u = graph.from_edges(base[-1], np.concatenate(src).astype(np.uint32), np.concatenate(dst).astype(np.uint32))
+    return u, base
+
+
+@pytest.mark.parametrize("variant", bfs_native.VARIANTS + ("learned",))
+def test_multi_root_union_matches_per_graph_bfs(gpu, variant):
+    rng = np.random.default_rng(17)
+    graphs = [graph.rmat_n(int(2 ** rng.uniform(6, 11)), int(rng.integers(4, 33)) * 64, seed=100 + i)
+              for i in range(24)]
+    u, base = _union(graphs)
+    dg = device.DeviceGraph(u)
+    eng = bfs_native.BFSEngine(dg)
+    roots = [base[i] + g.max_out_degree_vertex() for i, g in enumerate(graphs)]
+    if variant == "learned":
+        eng.set_model(switch.Model.init(rng.normal(6, 3, (64, 8)), seed=2, target_mean=8.0).w)
+        r = eng.run(roots, "learned", "td")
+    else:
+        r = eng.run(roots, "fixed", variant)
+    want_counts = np.zeros(r.num_levels, dtype=np.int64)
+    for i, g in enumerate(graphs):
+        want, _ = O.bfs_queue(O.CSR.from_graph(g), roots[i] - base[i])
+        assert np.array_equal(r.levels[base[i]:base[i + 1]], want), (variant, i)
+        reached = want[want != bfs_native.UNVISITED]
+        assert reached.max() + 1 <= r.num_levels
+        want_counts[:reached.max() + 1] += np.bincount(reached, minlength=reached.max() + 1)
+    # counts[d] = discovered at level d = vertices at depth d + 1.
+    assert np.array_equal(r.counts[:-1], want_counts[1:r.num_levels]), variant
+    assert r.counts[-1] == 0
+    eng.close()
+    dg.free()
+
+
+@pytest.mark.parametrize("hub,wave", [(256, 1), (512, 8), (100000, 1000)])
+def test_tuning_never_changes_depths(cases, hub, wave):
+    for g, csr, dg, eng, roots in cases[:3] + cases[5:7]:
+        eng.set_tuning(hub, wave)
+        try:
+            for v in bfs_native.VARIANTS:
+                r = eng.run(roots[0], "fixed", v)
+                want, counts = _want(csr, roots[0])
+                assert np.array_equal(r.levels, want), (g.name, hub, wave, v)
+                assert np.array_equal(r.counts, counts), (g.name, hub, wave, v)
+        finally:
+            eng.set_tuning()
+
+
+def test_level_limit_is_an_error(gpu):
+    g = graph.chain(100)
+    dg = device.DeviceGraph(g)
+    eng = bfs_native.BFSEngine(dg, max_levels=10)
+    with pytest.raises(N.BelewitteError):
+        eng.run(0, "fixed", "td")
+    eng.close()
+    dg.free()
+
+
+def test_rejects_bad_arguments(cases):
+    g, csr, dg, eng, roots = cases[0]
+    with pytest.raises(N.BelewitteError):
+        eng.run(g.vertex_count, "fixed", "td")
+    with pytest.raises(KeyError):
+        eng.run(0, "fixed", "nope")
+    fresh = bfs_native.BFSEngine(dg)
+    with pytest.raises(N.BelewitteError):
+        fresh.run(0, "learned", "td")  # no model set
+    fresh.close()
+    n = C.c_uint32()
+    assert N.hip_lib().bw_bfs_engine_run(eng._e, 0, 9, 0, 1, 1, 0, None, None, None, C.byref(n), None,
+                                         None) != 0
