@@ -179,6 +179,11 @@ _HIP_SIGS = {
     "bw_bfs_engine_begin_multi": (C.c_int, [vp, u32p, C.c_uint32, C.c_int32, C.c_int32, C.c_int32, vp]),
     "bw_bfs_engine_run_multi": (C.c_int, [vp, u32p, C.c_uint32, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
                                           C.c_int32, u32p, u64p, i32p, u32p, f32p, vp]),
+    # multi-source BFS
+    "bw_msbfs_create": (C.c_int, [C.POINTER(vp), _G, C.c_uint32]),
+    "bw_msbfs_destroy": (C.c_int, [vp]),
+    "bw_msbfs_set_tuning": (C.c_int, [vp, C.c_uint32]),
+    "bw_msbfs_run": (C.c_int, [vp, u32p, C.c_uint32, vp, C.c_int32, C.c_int32, u64p, u32p, f32p, vp]),
     # variant predictor
     "bw_mlp_params": (C.c_int32, [C.c_int32]),
     "bw_mlp_forward": (C.c_int, [vp, C.c_int32, vp, C.c_uint64, vp, vp]),

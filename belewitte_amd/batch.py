@@ -1,0 +1,103 @@
+"""Batched BFS and the autotuner of its work-division thresholds.
+
+Two batch shapes, each run as one device-driven level loop launched in
+captured hipGraph batches (no host round trip per level):
+
+* many sources on one graph: :class:`~belewitte_amd.bfs_native.MultiSourceBFS`
+  (64 searches share every pass over the graph);
+* many small graphs: their disjoint union (:func:`union`) searched once from
+  all their roots (:class:`~belewitte_amd.bfs_native.BFSEngine` multi-root);
+  each component gets exactly its own BFS depths.
+
+The autotuner times whole batch runs over a grid of the kernels'
+thread/wave/workgroup thresholds (hub out-degree for workgroup-sliced
+top-down expansion, in-degree for wave-wide bottom-up probes) and keeps
+the fastest; the thresholds never change the depths, only the time.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import graph as G
+from .bfs_native import UNVISITED, BFSEngine, MultiSourceBFS
+
+HUB_CANDIDATES = (1024, 4096, 16384)
+WAVE_CANDIDATES = (8, 16, 32, 64, 128)
+
+
+def union(graphs):
+    """Disjoint union of `graphs` (vertex ids of graph i shifted by
+    base[i]); returns (Graph, base) with base[-1] = total vertices."""
+    src, dst, base = [], [], [0]
+    for g in graphs:
+        s, d = g.edges()
+        src.append(s.astype(np.uint64) + np.uint64(base[-1]))
+        dst.append(d.astype(np.uint64) + np.uint64(base[-1]))
+        base.append(base[-1] + g.vertex_count)
+    if base[-1] >= 2 ** 32:
+        raise ValueError("the union exceeds 32-bit vertex ids")
+    s = np.concatenate(src).astype(np.uint32) if src else np.zeros(0, np.uint32)
+    d = np.concatenate(dst).astype(np.uint32) if dst else np.zeros(0, np.uint32)
+    u = G.from_edges(base[-1], s, d)
+    u.name = f"union-{len(graphs)}"
+    return u, base
+
+
+def small_rmat_batch(count: int = 256, seed: int = 7, log2_min: float = 10.0, log2_max: float = 16.0,
+                     ef_min: int = 4, ef_max: int = 32):
+    """`count` directed R-MAT graphs with vertex counts log-uniform in
+    [2^log2_min, 2^log2_max] and edge factors uniform in [ef_min, ef_max]."""
+    out = []
+    for i in range(count):
+        u1 = G.hash64(seed, 1, i) / 2.0 ** 64
+        u2 = G.hash64(seed, 2, i) / 2.0 ** 64
+        n = int(round(2.0 ** (log2_min + (log2_max - log2_min) * u1)))
+        ef = ef_min + int(u2 * (ef_max - ef_min + 1))
+        out.append(G.rmat_n(n, n * ef, seed=seed * 1000 + i))
+    return out
+
+
+@dataclass
+class Tuned:
+    params: tuple
+    ms: float
+    table: dict
+
+
+def autotune_engine(eng: BFSEngine, roots, policy: str = "fixed", variant="td", runs: int = 2,
+                    hubs=HUB_CANDIDATES, waves=WAVE_CANDIDATES) -> Tuned:
+    """Grid search of (hub_degree, wave_degree) on whole runs from `roots`
+    (a root, or a list for one multi-root search); leaves the engine set
+    to the fastest."""
+    table = {}
+    for h in hubs:
+        for w in waves:
+            eng.set_tuning(h, w)
+            best = math.inf
+            for _ in range(runs):
+                best = min(best, eng.run(roots, policy, variant, download=False).total_ms)
+            table[(h, w)] = best
+    params = min(table, key=table.get)
+    eng.set_tuning(*params)
+    return Tuned(params, table[params], {f"{h}/{w}": v for (h, w), v in table.items()})
+
+
+def autotune_msbfs(ms: MultiSourceBFS, sources, runs: int = 2, waves=WAVE_CANDIDATES + (256, 1 << 30)) -> Tuned:
+    table = {}
+    for w in waves:
+        ms.set_tuning(w)
+        best = math.inf
+        for _ in range(runs):
+            best = min(best, ms.run(sources, download=False)[1])
+        table[w] = best
+    w = min(table, key=table.get)
+    ms.set_tuning(w)
+    return Tuned((w,), table[w], {str(k): v for k, v in table.items()})
+
+
+def reached_edges(deg: np.ndarray, depths: np.ndarray) -> int:
+    """Out-degree sum over the reached vertices of one search."""
+    return int(deg[depths != UNVISITED].sum())

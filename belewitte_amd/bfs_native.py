@@ -201,6 +201,66 @@ class BFSEngine:
             pass
 
 
+class MultiSourceBFS:
+    """Up to 64 searches on one graph sharing every pass over it
+    (bw_msbfs_*): depths[i] is the BFS depth array from sources[i]."""
+
+    MAX_SOURCES = 64
+
+    def __init__(self, dg: DeviceGraph, max_levels: int | None = None):
+        if dg.offset_bytes != 4:
+            raise ValueError("multi-source BFS reads uint32 offsets")
+        dg.ensure("rev_csr")
+        self.dg = dg
+        self.max_levels = int(max_levels or min(dg.vertex_count + 1, 1 << 22))
+        p = C.c_void_p()
+        N.check_hip(N.hip_lib().bw_msbfs_create(C.byref(p), dg.ref, self.max_levels))
+        self._m = p
+        self._depths = None
+        self._bits = np.zeros(self.max_levels, dtype=np.uint64)
+        self.wave_degree = 32
+
+    def set_tuning(self, wave_degree: int) -> None:
+        N.check_hip(N.hip_lib().bw_msbfs_set_tuning(self._m, int(wave_degree)))
+        self.wave_degree = int(wave_degree)
+
+    def run(self, sources, levels_per_sync: int = 8, use_graph: bool = True, download: bool = True, stream=None):
+        """Returns (depths int32[n][V] or None, total_ms, level_bits)."""
+        from .device import DeviceArray
+        src = np.ascontiguousarray(sources, dtype=np.uint32)
+        n, V = src.size, self.dg.vertex_count
+        if not 1 <= n <= self.MAX_SOURCES:
+            raise ValueError("multi-source BFS takes 1..64 sources")
+        if self._depths is None or self._depths.count < n * V:
+            if self._depths is not None:
+                self._depths.free()
+            self._depths = DeviceArray(self.MAX_SOURCES * V, np.int32)
+        nl, ms = C.c_uint32(), C.c_float()
+        N.check_hip(N.hip_lib().bw_msbfs_run(self._m, src.ctypes.data_as(N.u32p), n, self._depths.ptr,
+                                             int(levels_per_sync), int(bool(use_graph)),
+                                             self._bits.ctypes.data_as(N.u64p), C.byref(nl), C.byref(ms), stream))
+        depths = None
+        if download:
+            flat = np.empty(n * V, dtype=np.int32)
+            N.check_hip(N.hip_lib().bw_hip_memcpy_d2h(flat.ctypes.data_as(C.c_void_p), self._depths.ptr, 4 * n * V))
+            depths = flat.reshape(n, V)
+        return depths, float(ms.value), self._bits[:min(int(nl.value), self.max_levels)].copy()
+
+    def close(self) -> None:
+        if self._m:
+            N.hip_lib().bw_msbfs_destroy(self._m)
+            self._m = None
+        if self._depths is not None:
+            self._depths.free()
+            self._depths = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def teps_edges(g, levels: np.ndarray) -> int:
     """Edges counted by the TEPS figures: the out-degree sum over reached
     vertices (directed), halved for undirected graphs (Graph500)."""

@@ -20,7 +20,6 @@ benchmark-analysis/model-src/Evaluate.hs:229-230).  Here:
 """
 from __future__ import annotations
 
-import ctypes as C
 import json
 from dataclasses import dataclass
 from pathlib import Path
@@ -28,7 +27,7 @@ from pathlib import Path
 import numpy as np
 
 from . import _native as N
-from .bfs_native import NUM_FEATURES, VARIANTS, BFSEngine, LevelRecord, teps_edges
+from .bfs_native import NUM_FEATURES, VARIANTS, BFSEngine, teps_edges
 from .device import DeviceArray
 
 HIDDEN = 32
@@ -263,3 +262,72 @@ def evaluate(engine: BFSEngine, g, sources, model: Model, repeats: int = 1, runs
         "depth_mismatches": mismatches,
         "learned_choice_histogram": np.bincount(learned_choice, minlength=K).tolist(),
     }
+
+
+MODEL_PATH = Path(__file__).resolve().parent / "data" / "bfs_switch_model.json"
+
+
+def training_graphs(small: bool = False):
+    """The training inputs: an Erdos-Renyi graph (2^24 vertices, average
+    degree 16, undirected), a 4096 x 4096 4-neighbour grid with 1 % of its
+    edges rewired, and the directed R-MAT scale-22 graph of the PageRank
+    workload.  small=True shrinks each for quick runs."""
+    from . import graph as G
+    if small:
+        yield G.erdos_renyi(1 << 16, 1 << 19, seed=3, undirected=True)
+        yield G.grid2d(256, 256, rewire=0.01, seed=4)
+        yield G.rmat(16, 16, seed=2)
+        return
+    yield G.erdos_renyi(1 << 24, 134217728, seed=3, undirected=True)
+    yield G.grid2d(4096, 4096, rewire=0.01, seed=4)
+    yield G.rmat(22, 16, seed=2)
+
+
+def main(argv=None) -> int:
+    import argparse
+    import sys
+    import time
+
+    from . import device
+    from . import graph as G
+    ap = argparse.ArgumentParser(description="collect exhaustive per-level BFS timings and train the switch")
+    ap.add_argument("--out", default=str(MODEL_PATH))
+    ap.add_argument("--samples-out", default=None)
+    ap.add_argument("--sources", type=int, default=16)
+    ap.add_argument("--epochs", type=int, default=3000)
+    ap.add_argument("--small", action="store_true")
+    args = ap.parse_args(argv)
+    device.require_gpu()
+    device.set_device(0)
+    samples = None
+    for g in training_graphs(args.small):
+        t0 = time.perf_counter()
+        dg = device.DeviceGraph(g)
+        eng = BFSEngine(dg)
+        src = G.random_sources(g, args.sources, seed=5)
+        s, _ = collect(eng, src)
+        print(f"[switch] {g.name}: {s.X.shape[0]} levels from {len(src)} sources in "
+              f"{time.perf_counter() - t0:.1f}s; best-fixed totals {s.ms.sum(axis=0).round(3).tolist()} ms, "
+              f"oracle {s.ms.min(axis=1).sum():.3f} ms", file=sys.stderr, flush=True)
+        samples = s if samples is None else samples.concat(s)
+        eng.close()
+        dg.free()
+    if args.samples_out:
+        samples.save(args.samples_out)
+    model, loss = fit(samples, epochs=args.epochs)
+    choice = model.choose(samples.X)
+    sim = simulate(samples, choice)
+    fixed = np.sum([v["fixed"] for v in sim.values()], axis=0)
+    learned = sum(v["switch"] for v in sim.values())
+    oracle = sum(v["oracle"] for v in sim.values())
+    print(f"[switch] trained: loss {loss[0]:.4f} -> {loss[-1]:.4f}; training-set totals: oracle {oracle:.3f} ms, "
+          f"learned {learned:.3f} ms, best fixed {fixed.min():.3f} ms ({VARIANTS[int(np.argmin(fixed))]})",
+          file=sys.stderr, flush=True)
+    Path(args.out).parent.mkdir(parents=True, exist_ok=True)
+    model.save(args.out)
+    print(args.out)
+    return 0
+
+
+if __name__ == "__main__":
+    raise SystemExit(main())

@@ -62,6 +62,93 @@ def cpu_baseline(g, ranks_gpu, threads: int):
                       f"(uniform dangling) over all {g.edge_count} edges, {dt:.2f} s"}, err, csr
 
 
+def bfs_learned_switch(scale: int, sources: int) -> dict:
+    """Undirected R-MAT (scale, edge factor 16, seed 5), `sources` random
+    sources of out-degree >= 1: the learned per-level switch against the
+    best fixed native variant and the per-level oracle, simulated on
+    exhaustive per-level timings and executed end to end on the device."""
+    from belewitte_amd import bfs_native, device, graph, switch
+    t0 = time.perf_counter()
+    g = graph.rmat(scale, 16, seed=5, undirected=True)
+    log(f"bfs graph {g} built in {time.perf_counter() - t0:.1f}s")
+    dg = device.DeviceGraph(g)
+    eng = bfs_native.BFSEngine(dg)
+    src = graph.random_sources(g, sources, seed=8)
+    if switch.MODEL_PATH.exists():
+        model, origin = switch.Model.load(switch.MODEL_PATH), f"{switch.MODEL_PATH.name} (trained offline on the GPU)"
+    else:
+        samples, _ = switch.collect(eng, src[:4])
+        model, _ = switch.fit(samples)
+        origin = "trained in this run on 4 of the sources (no stored model)"
+    t0 = time.perf_counter()
+    res = switch.evaluate(eng, g, src, model, repeats=2, runs=3)
+    log(f"bfs switch evaluated in {time.perf_counter() - t0:.1f}s: {res['executed_ms']}")
+    # Algorithmic bytes of a top-down pass per source: 8 E_r + 16 V_r.
+    deg = np.diff(g.offsets()).astype(np.int64)
+    V_r = E_r = 0
+    for s in src:
+        r = eng.run(int(s), "learned", 0)
+        reached = r.levels != bfs_native.UNVISITED
+        V_r += int(reached.sum())
+        E_r += int(deg[reached].sum())
+    t = res["executed_ms"]["learned"] / 1e3
+    res["roofline"] = {"bound": "hbm", "algorithmic_bytes": 8 * E_r + 16 * V_r,
+                       "achieved_gbs": round((8 * E_r + 16 * V_r) / t / 1e9, 1) if t else None,
+                       "peak_gbs": HBM_PEAK_GBS}
+    res["graph"] = {"name": g.name, "vertices": g.vertex_count, "stored_edges": g.edge_count,
+                    "teps_convention": "undirected: reached out-degree sum / 2 (Graph500)"}
+    res["model"] = origin
+    eng.close()
+    dg.free()
+    return res
+
+
+def bfs_batched() -> dict:
+    """Batch 1: 64 sources on a directed R-MAT scale 20, edge factor 16
+    (seed 6) in one multi-source search; batch 2: 256 directed R-MAT graphs
+    of 2^10..2^16 vertices (log-uniform) and edge factors 4..32 searched as
+    one disjoint union from their max-out-degree vertices.  Both autotuned,
+    both one device-driven level loop in hipGraph batches."""
+    from belewitte_amd import batch, bfs_native, device, graph
+    out = {}
+    g = graph.rmat(20, 16, seed=6)
+    dg = device.DeviceGraph(g)
+    src = graph.random_sources(g, 64, seed=9)
+    ms = bfs_native.MultiSourceBFS(dg)
+    tuned = batch.autotune_msbfs(ms, src)
+    depths, t, _ = ms.run(src)
+    best = min(t, *(ms.run(src, download=False)[1] for _ in range(3)))
+    deg = np.diff(g.offsets()).astype(np.int64)
+    edges = sum(batch.reached_edges(deg, depths[i]) for i in range(len(src)))
+    # Spot check against single-source searches of the engine.
+    eng = bfs_native.BFSEngine(dg)
+    agree = all(np.array_equal(eng.run(int(src[i]), "fixed", "bu-wave").levels, depths[i]) for i in (0, 31, 63))
+    eng.close()
+    out["sources_64_rmat_s20"] = {"ms": round(best, 3), "gteps": round(edges / (best / 1e3) / 1e9, 2),
+                                  "wave_degree": tuned.params[0], "tuning_ms": tuned.table,
+                                  "reached_edges": edges, "spot_check_vs_single_source": bool(agree)}
+    ms.close()
+    dg.free()
+
+    graphs = batch.small_rmat_batch(256, seed=7)
+    u, base = batch.union(graphs)
+    roots = [base[i] + gi.max_out_degree_vertex() for i, gi in enumerate(graphs)]
+    dg = device.DeviceGraph(u)
+    eng = bfs_native.BFSEngine(dg)
+    tuned = batch.autotune_engine(eng, roots, "fixed", "td")
+    best = min(eng.run(roots, "fixed", "td", download=False).total_ms for _ in range(5))
+    r = eng.run(roots, "fixed", "td")
+    deg = np.diff(u.offsets()).astype(np.int64)
+    edges = batch.reached_edges(deg, r.levels)
+    out["graphs_256_rmat"] = {"ms": round(best, 3), "gteps": round(edges / (best / 1e3) / 1e9, 2),
+                              "vertices": u.vertex_count, "edges": u.edge_count, "reached_edges": edges,
+                              "levels": r.num_levels, "tuned_hub_wave": list(tuned.params),
+                              "tuning_ms": tuned.table}
+    eng.close()
+    dg.free()
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -69,6 +156,9 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-variants", action="store_true", help="skip the per-variant table")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-bfs", action="store_true", help="skip the BFS figures")
+    ap.add_argument("--bfs-scale", type=int, default=25, help="R-MAT scale of the learned-switch BFS")
+    ap.add_argument("--bfs-sources", type=int, default=16)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
@@ -172,10 +262,14 @@ def main() -> int:
             log(f"variant {name}: {table[name]} GE/s/iter")
         table[pagerank.FUSED] = round(value, 3)
         out["variants_ge_s_iter"] = table
-
-    print(json.dumps(out), flush=True)
     fused.close()
     dg.free()
+
+    if not args.no_bfs:
+        out["bfs_learned_switch"] = bfs_learned_switch(args.bfs_scale, args.bfs_sources)
+        out["bfs_batched"] = bfs_batched()
+
+    print(json.dumps(out), flush=True)
     return 0
 
 

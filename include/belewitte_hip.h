@@ -512,6 +512,31 @@ int bw_bfs_engine_run_multi(bw_bfs_engine *engine, const uint32_t *roots,
                             uint32_t *num_levels, float *total_ms,
                             void *stream);
 
+/* ------------------------------------------- multi-source BFS (MI355X) */
+
+/* Up to 64 BFS searches on one graph sharing every pass over it (64-bit
+ * visit/frontier masks per vertex, bottom-up levels with early exit, the
+ * level loop on the device).  Search i computes the depths of
+ * bfs/bfs.cpp:43-118 from sources[i].  Needs the reverse CSR with uint32
+ * offsets (bw_graph_dev.rev_offsets / rev_cols). */
+typedef struct bw_msbfs bw_msbfs;
+
+#define BW_MSBFS_DEFAULT_WAVE_DEGREE 32
+
+int bw_msbfs_create(bw_msbfs **ms, const bw_graph_dev *g, uint32_t max_levels);
+int bw_msbfs_destroy(bw_msbfs *ms);
+/* In-degree above which a whole wave probes a vertex's in-neighbours. */
+int bw_msbfs_set_tuning(bw_msbfs *ms, uint32_t wave_degree);
+/* depths: device int32[n][V], row i = depths from sources[i] (host array,
+ * 1 <= n <= 64).  Levels are launched in batches of levels_per_sync
+ * (replayed as a captured hipGraph if use_graph).  Optional host outputs:
+ * level_bits (uint64[max_levels], newly reached (search, vertex) pairs per
+ * level), total_ms (device time of the whole run). */
+int bw_msbfs_run(bw_msbfs *ms, const uint32_t *sources, uint32_t n,
+                 int32_t *depths, int32_t levels_per_sync, int32_t use_graph,
+                 uint64_t *level_bits, uint32_t *num_levels, float *total_ms,
+                 void *stream);
+
 /* ------------------------------------------ the variant predictor (MLP) */
 
 /* FP32 MLP 8 -> 32 -> 32 -> K in the packed layout of

@@ -259,3 +259,39 @@ def test_rejects_bad_arguments(cases):
     n = C.c_uint32()
     assert N.hip_lib().bw_bfs_engine_run(eng._e, 0, 9, 0, 1, 1, 0, None, None, None, C.byref(n), None,
                                          None) != 0
+
+
+@pytest.mark.parametrize("nsrc", [1, 17, 64])
+@pytest.mark.parametrize("use_graph", [False, True])
+def test_multi_source_bit_exact(cases, nsrc, use_graph):
+    rng = np.random.default_rng(nsrc)
+    for g, csr, dg, eng, roots in cases:
+        ms = bfs_native.MultiSourceBFS(dg)
+        try:
+            src = rng.integers(0, g.vertex_count, nsrc)
+            src[0] = roots[0]
+            if nsrc > 2:
+                src[1] = src[2]  # a repeated source
+            depths, ms_time, bits = ms.run(src, use_graph=use_graph, levels_per_sync=3)
+            total = 0
+            for i, s in enumerate(src):
+                want, _ = O.bfs_queue(csr, int(s))
+                assert np.array_equal(depths[i], want), (g.name, i, int(s))
+                total += int(np.count_nonzero(want != bfs_native.UNVISITED)) - 1
+            assert int(bits.sum()) == total and bits[-1] == 0
+            assert ms_time > 0
+        finally:
+            ms.close()
+
+
+@pytest.mark.parametrize("wave", [0, 4, 1 << 20])
+def test_multi_source_tuning(cases, wave):
+    g, csr, dg, eng, roots = cases[0]
+    ms = bfs_native.MultiSourceBFS(dg)
+    ms.set_tuning(wave)
+    src = np.arange(0, g.vertex_count, g.vertex_count // 64)[:64]
+    depths, _, _ = ms.run(src)
+    for i, s in enumerate(src):
+        want, _ = O.bfs_queue(csr, int(s))
+        assert np.array_equal(depths[i], want), (wave, i)
+    ms.close()
