@@ -6,13 +6,20 @@
 // with the reference kernels' signatures that call the gfx950 kernels of
 // libbelewitte_hip through its C ABI.
 
+#include <dlfcn.h>
+
+#include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <fstream>
+#include <iterator>
 #include <limits>
+#include <memory>
 #include <string>
 #include <vector>
 
 #include "Algorithm.hpp"
+#include "belewitte_graph.h"
 #include "HIPBackend.hpp"
 #include "ImplementationTemplate.hpp"
 #include "Timer.hpp"
@@ -230,6 +237,175 @@ static auto insertVariant()
     return kernelMap;
 }
 
+namespace {
+
+// Reads the "weights" array of a switch model file (the JSON written by
+// belewitte_amd.switch.Model.save).
+std::vector<float> readModel(const std::string &path, int32_t outputs)
+{
+    std::ifstream in(path);
+    checkError(in.good(), "Cannot open model file '", path, "'");
+    const std::string text((std::istreambuf_iterator<char>(in)), std::istreambuf_iterator<char>());
+    checkError(text.find("\"belewitte-bfs-mlp\"") != std::string::npos, "'", path, "' is not a BFS switch model");
+    const size_t key = text.find("\"weights\"");
+    const size_t open = key == std::string::npos ? key : text.find('[', key);
+    checkError(open != std::string::npos, "'", path, "' has no weights");
+    std::vector<float> w;
+    const char *p = text.c_str() + open + 1;
+    while (*p) {
+        while (*p == ' ' || *p == ',' || *p == '\n' || *p == '\r' || *p == '\t') p++;
+        if (*p == ']') break;
+        char *end = nullptr;
+        const float x = std::strtof(p, &end);
+        checkError(end != p, "malformed weight in '", path, "'");
+        w.push_back(x);
+        p = end;
+    }
+    checkError(static_cast<int32_t>(w.size()) == bw_bfs_engine_mlp_len(outputs), "'", path, "' holds ",
+               w.size(), " weights, expected ", bw_bfs_engine_mlp_len(outputs));
+    return w;
+}
+
+// Default model: data/bfs_switch_model.json two directories above this
+// library (belewitte_amd/lib/kernels -> belewitte_amd/data).
+std::string defaultModelPath()
+{
+    Dl_info info;
+    if (!dladdr(reinterpret_cast<void *>(&defaultModelPath), &info) || !info.dli_fname) return "";
+    std::string so = info.dli_fname;
+    const size_t slash = so.rfind('/');
+    const std::string dir = slash == std::string::npos ? "." : so.substr(0, slash);
+    return dir + "/../../data/bfs_switch_model.json";
+}
+
+int32_t nativeVariant(const std::string &name)
+{
+    if (name == "td") return BW_BFS_NATIVE_TD;
+    if (name == "bu") return BW_BFS_NATIVE_BU;
+    if (name == "bu-wave") return BW_BFS_NATIVE_BU_WAVE;
+    reportError("Unknown native BFS variant '", name, "' (td, bu, bu-wave)");
+}
+
+}  // namespace
+
+// The MI355X-native searches, which have no counterpart in the reference:
+// the device-driven engine of libbelewitte_hip (level loop on the device,
+// per-level variant chosen on the device).  Same lifecycle, options (-r)
+// and output (`i\t<depth>`) as the BFS implementations above; timers
+// "computation" (device time of the search) and "<n>:bfsLevel" (device
+// time of each level).
+struct NativeBFS : public ImplementationBase {
+    int32_t policy;
+    std::string variant = "td";
+    std::string modelPath;
+    std::string schedule;
+    unsigned root = 0;
+    size_t levelsPerSync = 8;
+    bw_graph *host = nullptr;
+    bw_graph_dev dev{};
+    std::vector<void *> bufs;
+
+    explicit NativeBFS(int32_t p) : policy(p)
+    {
+        options.add('r', "root", "NUM", root, "Starting vertex for BFS.");
+        options.add("variant", "NAME", variant, "Fixed variant (td, bu, bu-wave), also the learned fallback.");
+        options.add("model", "FILE", modelPath, "Switch model (default: the one shipped with the library).");
+        options.add("schedule", "LIST", schedule, "Comma-separated per-level variants, repeated.");
+        options.add("levels-per-sync", "NUM", levelsPerSync, "Levels launched between done-flag reads.");
+    }
+
+    ~NativeBFS() override { NativeBFS::freeGraph(); }
+
+    void *upload(const void *src, size_t bytes)
+    {
+        void *d = nullptr;
+        BW_CHK(bw_hip_dev_alloc(&d, std::max<size_t>(bytes, 4)));
+        bufs.push_back(d);
+        if (bytes) BW_CHK(bw_hip_memcpy_h2d(d, src, bytes));
+        return d;
+    }
+
+    void loadGraph(const std::string file) override
+    {
+        freeGraph();
+        if (bw_graph_load(&host, file.c_str()) != 0) reportError("Failed to load '", file, "': ", bw_graph_last_error());
+        const uint64_t V = bw_graph_vertex_count(host), E = bw_graph_edge_count(host);
+        std::vector<uint32_t> off(V + 1);
+        dev.vertex_count = V;
+        dev.edge_count = E;
+        dev.offset_bytes = 4;
+        for (int rev = 0; rev < 2; rev++) {
+            if (bw_graph_offsets32(host, rev, off.data()) != 0) reportError(bw_graph_last_error());
+            void *o = upload(off.data(), 4 * (V + 1));
+            auto *c = static_cast<uint32_t *>(upload(bw_graph_cols(host, rev), 4 * E));
+            if (rev) {
+                dev.rev_offsets = o;
+                dev.rev_cols = c;
+            } else {
+                dev.offsets = o;
+                dev.cols = c;
+            }
+        }
+    }
+
+    void freeGraph() override
+    {
+        for (void *p : bufs) (void)bw_hip_dev_free(p);
+        bufs.clear();
+        if (host) bw_graph_free(host);
+        host = nullptr;
+        dev = bw_graph_dev{};
+    }
+
+    void runImplementation(std::ofstream &outputFile) override
+    {
+        const uint64_t V = dev.vertex_count;
+        if (root >= V) return;
+        const uint32_t maxLevels = static_cast<uint32_t>(std::min<uint64_t>(V + 1, 1u << 22));
+        bw_bfs_engine *e = nullptr;
+        BW_CHK(bw_bfs_engine_create(&e, &dev, nullptr, maxLevels));
+        const int32_t fixed = nativeVariant(variant);
+        int32_t slen = 1;
+        if (policy == BW_POLICY_LEARNED) {
+            const auto w = readModel(modelPath.empty() ? defaultModelPath() : modelPath, BW_BFS_NATIVE_VARIANTS);
+            BW_CHK(bw_bfs_engine_set_model(e, w.data(), BW_BFS_NATIVE_VARIANTS));
+        } else if (policy == BW_POLICY_SCHEDULE) {
+            std::vector<int32_t> s;
+            size_t start = 0;
+            while (start <= schedule.size()) {
+                const size_t comma = schedule.find(',', start);
+                s.push_back(nativeVariant(schedule.substr(start, comma - start)));
+                if (comma == std::string::npos) break;
+                start = comma + 1;
+            }
+            BW_CHK(bw_bfs_engine_set_schedule(e, s.data(), static_cast<int32_t>(s.size())));
+            slen = static_cast<int32_t>(s.size());
+        }
+        Timer computation("computation", run_count);
+        std::vector<std::unique_ptr<Timer>> levelTimers;
+        std::vector<uint32_t> counts(maxLevels);
+        std::vector<uint64_t> times(maxLevels);
+        uint32_t levels = 0;
+        for (size_t i = 0; i < run_count; i++) {
+            float ms = 0;
+            BW_CHK(bw_bfs_engine_run(e, root, policy, fixed, slen, static_cast<int32_t>(levelsPerSync), 1,
+                                     counts.data(), times.data(), nullptr, &levels, &ms, nullptr));
+            computation.add(static_cast<double>(ms) * 1e6);
+            for (uint32_t d = 0; d < levels; d++) {
+                if (d >= levelTimers.size())
+                    levelTimers.push_back(std::make_unique<Timer>(std::to_string(d) + ":bfsLevel", run_count));
+                levelTimers[d]->add(static_cast<double>(times[d]));
+            }
+        }
+        int32_t *devLevels = nullptr;
+        BW_CHK(bw_bfs_engine_levels(e, &devLevels));
+        std::vector<int32_t> depth(V);
+        BW_CHK(bw_hip_memcpy_d2h(depth.data(), devLevels, 4 * V));
+        BW_CHK(bw_bfs_engine_destroy(e));
+        for (uint64_t v = 0; v < V; v++) outputFile << v << "\t" << depth[v] << "\n";
+    }
+};
+
 extern "C" register_algorithm_t registerHIP;
 extern "C" void registerHIP(Algorithm &result)
 {
@@ -240,4 +416,7 @@ extern "C" void registerHIP(Algorithm &result)
     kernelMap += insertVariant<blockreduce>();
     for (auto &[name, kernel] : kernelMap) result.addImplementation(name, make_implementation<BFS>(kernel));
     result.addImplementation("switch", make_switch_implementation<BFS>(kernelMap));
+    result.addImplementation("native", std::make_unique<NativeBFS>(BW_POLICY_FIXED));
+    result.addImplementation("native-schedule", std::make_unique<NativeBFS>(BW_POLICY_SCHEDULE));
+    result.addImplementation("native-learned", std::make_unique<NativeBFS>(BW_POLICY_LEARNED));
 }

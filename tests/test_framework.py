@@ -34,7 +34,8 @@ def test_list_algorithms():
 def test_bfs_implementation_names():
     r = run("list", "implementations", "-a", "bfs")
     assert r.returncode == 0
-    assert r.stdout.split() == sorted(list(bfs.IMPLEMENTATIONS) + ["switch"])
+    assert r.stdout.split() == sorted(list(bfs.IMPLEMENTATIONS) + ["switch", "native", "native-learned",
+                                                                   "native-schedule"])
 
 
 def test_pagerank_implementation_names():

@@ -146,3 +146,25 @@ def test_switch_with_predictor_and_log(gfile, tmp_path):
     # Without a model the switch runs edge-list.
     run(["-a", "bfs", "-k", "switch", "-r", str(root), "-o", str(tmp_path), str(p)])
     assert np.array_equal(parse_bfs((tmp_path / "rmat11.output").read_text()), want.astype(np.int64))
+
+
+def test_native_engine_through_the_runner(gfile, tmp_path):
+    """The device-driven engine as kernel-runner implementations: fixed
+    variants, a schedule and the learned switch with a model file."""
+    from belewitte_amd import switch
+    g, p = gfile
+    root = g.max_out_degree_vertex()
+    want, nlev = O.bfs_queue(O.CSR.from_graph(g), root)
+    model = tmp_path / "m.json"
+    switch.Model.init(np.random.default_rng(0).normal(8, 3, (32, 8)), seed=1, target_mean=9.0).save(model)
+    jobs = [("native", ["--variant", "td"]), ("native", ["--variant", "bu-wave"]),
+            ("native-schedule", ["--schedule", "bu,td,bu-wave"]), ("native-learned", ["--model", str(model)])]
+    for i, (name, extra) in enumerate(jobs):
+        out = tmp_path / str(i)
+        out.mkdir()
+        run(["-a", "bfs", "-k", name, "-r", str(root), *extra, "-o", str(out), str(p)])
+        got = parse_bfs((out / "rmat11.output").read_text())
+        assert np.array_equal(got, want.astype(np.int64)), (name, extra)
+        timings = (out / "rmat11.timings").read_text().splitlines()
+        assert all(TIMING.match(t) for t in timings), timings
+        assert sum(1 for t in timings if ":bfsLevel:" in t) == nlev
