@@ -117,14 +117,19 @@ struct Emitter {
     double sq = 0.0;
 
     // Called by the lanes that discovered a vertex (any subset of a wave).
-    __device__ __forceinline__ void emit(const Graph &g, const Buffers &b, uint32_t d, uint32_t w)
+    // mark: set w's bits in the next-frontier and visited bitmaps here
+    // (top-down, where any workgroup may discover any vertex); bottom-up
+    // levels set whole words per wave instead (see level_bu).
+    __device__ __forceinline__ void emit(const Graph &g, const Buffers &b, uint32_t d, uint32_t w, bool mark)
     {
         const uint32_t nq = (d & 1) ^ 1;
         const uint32_t dw = out_degree(g, w);
         outsum += dw;
         sq += static_cast<double>(dw) * static_cast<double>(dw);
-        atomicOr(&b.bits[(d + 1) % 3][w >> 5], 1u << (w & 31));
-        atomicOr(&b.vis[w >> 5], 1u << (w & 31));
+        if (mark) {
+            atomicOr(&b.bits[(d + 1) % 3][w >> 5], 1u << (w & 31));
+            atomicOr(&b.vis[w >> 5], 1u << (w & 31));
+        }
         if (dw > b.hub_degree) {
             // Slots are handed out by a counter that only grows, so every
             // slot below min(count, kHubCap) is written exactly once.
@@ -200,7 +205,7 @@ __device__ void level_td(const Graph &g, const Buffers &b, uint32_t d, Emitter &
                 }
                 const uint32_t u = b.hubq[cur][lo];
                 const uint32_t w = g.col[g.off[u] + static_cast<uint32_t>(k - b.hub_prefix[lo])];
-                if (try_discover(b.levels, w, nd)) em.emit(g, b, d, w);
+                if (try_discover(b.levels, w, nd)) em.emit(g, b, d, w, true);
             }
             em.flush(b, d, kBlock);
         }
@@ -241,7 +246,7 @@ __device__ void level_td(const Graph &g, const Buffers &b, uint32_t d, Emitter &
                 }
                 const uint64_t before = lo ? sscan[lo - 1] : 0;
                 const uint32_t w = g.col[sstart[lo] + static_cast<uint32_t>(k - before)];
-                if (try_discover(b.levels, w, nd)) em.emit(g, b, d, w);
+                if (try_discover(b.levels, w, nd)) em.emit(g, b, d, w, true);
             }
             em.flush(b, d, kBlock);
         }
@@ -266,11 +271,11 @@ __device__ void level_bu(const Graph &g, const Buffers &b, uint32_t d, Emitter &
             e = g.roff[v + 1];
         }
         const bool big = wave && e - s > b.wave_degree;
+        bool discovered = false;
         if (unvisited && !big) {
             for (uint32_t j = s; j < e; j++) {
                 if (test_bit(front, g.rcol[j])) {
-                    b.levels[v] = nd;
-                    em.emit(g, b, d, v);
+                    discovered = true;
                     break;
                 }
             }
@@ -287,10 +292,24 @@ __device__ void level_bu(const Graph &g, const Buffers &b, uint32_t d, Emitter &
                     const bool hit = j < ve && test_bit(front, g.rcol[j]);
                     found = __ballot(hit) != 0;
                 }
-                if (found && lane == l) {
-                    b.levels[v] = nd;
-                    em.emit(g, b, d, v);
-                }
+                if (found && lane == l) discovered = true;
+            }
+        }
+        if (discovered) {
+            b.levels[v] = nd;
+            em.emit(g, b, d, v, false);
+        }
+        // The wave's 64 vertices are two whole bitmap words that no other
+        // wave touches in a bottom-up level: plain stores, no atomics.  The
+        // next-frontier words were cleared two levels ago.
+        const uint64_t fm = __ballot(discovered);
+        if (fm) {
+            const uint64_t w0 = (base + (threadIdx.x & ~static_cast<uint32_t>(kWave - 1))) >> 5;
+            const uint32_t half = lane == 0 ? static_cast<uint32_t>(fm) : static_cast<uint32_t>(fm >> 32);
+            if ((lane == 0 || lane == 32) && half) {
+                const uint64_t wi = w0 + (lane >> 5);
+                b.bits[(d + 1) % 3][wi] = half;
+                b.vis[wi] |= half;
             }
         }
         em.flush(b, d, kBlock);

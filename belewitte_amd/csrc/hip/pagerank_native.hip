@@ -83,7 +83,7 @@ struct IterArgs {
     const LongRow *lrows;
     const uint4 *tiles;      // (first row, end row, edge begin, edge end)
     float2 *partials;        // one Kahan partial per piece
-    uint32_t npieces, nlong;
+    uint32_t npieces, nlong, ntiles;
     const double2 *prev;     // per-workgroup (residual, dangling) of the previous launch
     uint32_t nprev;
     double *resid_out;       // reduced residual of the previous launch
@@ -117,7 +117,12 @@ __device__ __forceinline__ void finalise(const IterArgs &a, uint32_t v, float su
 
 // Contributions of edges [e, e_end) that fall to this thread (edge
 // e + tid + kBlock*k for k < kPerThread): all column loads, then all gathers.
-__device__ __forceinline__ void gather_tile(const IterArgs &a, uint32_t e, uint32_t e_end, float (&val)[kPerThread])
+// With kHot > 0, sources below kHot are read from the workgroup's LDS copy
+// of their contributions (`hot`) instead of through the L2: on a graph
+// relabelled by decreasing out-degree those are the most gathered sources.
+template<uint32_t kHot>
+__device__ __forceinline__ void gather_tile(const IterArgs &a, uint32_t e, uint32_t e_end, float (&val)[kPerThread],
+                                            const float *hot)
 {
     uint32_t col[kPerThread];
 #pragma unroll
@@ -126,7 +131,10 @@ __device__ __forceinline__ void gather_tile(const IterArgs &a, uint32_t e, uint3
         col[k] = j < e_end ? __builtin_nontemporal_load(&a.rcols[j]) : 0xFFFFFFFFu;
     }
 #pragma unroll
-    for (int k = 0; k < kPerThread; k++) val[k] = col[k] != 0xFFFFFFFFu ? a.cin[col[k]] : 0.0f;
+    for (int k = 0; k < kPerThread; k++) {
+        if (kHot > 0 && col[k] < kHot) val[k] = hot[col[k]];
+        else val[k] = col[k] != 0xFFFFFFFFu ? a.cin[col[k]] : 0.0f;
+    }
 }
 
 // Block-wide (sum, compensation) merge in a fixed order; valid in thread 0.
@@ -193,7 +201,7 @@ __global__ __launch_bounds__(kBlock) void pr_pieces_kernel(IterArgs a)
     }
     const uint4 pc = a.pieces[blockIdx.x];
     float val[kPerThread];
-    gather_tile(a, pc.z, pc.w, val);
+    gather_tile<0>(a, pc.z, pc.w, val, nullptr);
     Kahan k;
 #pragma unroll
     for (int q = 0; q < kPerThread; q++) k.add(val[q]);
@@ -202,6 +210,11 @@ __global__ __launch_bounds__(kBlock) void pr_pieces_kernel(IterArgs a)
     if (threadIdx.x == 0) a.partials[pc.y] = make_float2(s, c);
 }
 
+// Work items: nlong long-row finalisers, then the row tiles.  Workgroup b
+// takes items b, b + gridDim.x, ... (one item each when the grid covers
+// them all; persistent otherwise, which also loads the kHot table once per
+// workgroup rather than once per tile).
+template<uint32_t kHot>
 __global__ __launch_bounds__(kBlock) void pr_rows_kernel(IterArgs a)
 {
     __shared__ float vals[kTileNnz];
@@ -211,15 +224,22 @@ __global__ __launch_bounds__(kBlock) void pr_rows_kernel(IterArgs a)
     __shared__ uint32_t bigRows[kTileNnz / kBigRow];
     __shared__ uint32_t nbig;
     __shared__ double red[kBlock / kWave][2];
+    __shared__ float hot[kHot > 0 ? kHot : 1];
 
     const int tid = threadIdx.x;
     const int lane = tid & (kWave - 1), wid = tid / kWave;
     const double D = (a.mode == BW_PR_DANGLING_UNIFORM) ? *a.dangl : 0.0;
     double resid = 0.0, dangl = 0.0;
+    if (kHot > 0) {
+        const uint32_t n = min(kHot, a.V);
+        for (uint32_t i = tid; i < n; i += kBlock) hot[i] = a.cin[i];
+        __syncthreads();
+    }
 
-    if (blockIdx.x < a.nlong) {
+    for (uint32_t item = blockIdx.x; item < a.nlong + a.ntiles; item += gridDim.x) {
+    if (item < a.nlong) {
         if (tid == 0) {
-            const LongRow lr = a.lrows[blockIdx.x];
+            const LongRow lr = a.lrows[item];
             float s = 0.0f, c = 0.0f;
             for (uint32_t q = 0; q < lr.pieces; q++) {
                 const float2 p = a.partials[lr.base + q];
@@ -228,10 +248,10 @@ __global__ __launch_bounds__(kBlock) void pr_rows_kernel(IterArgs a)
             finalise(a, lr.row, s - c, a.deg[lr.row], a.cin[lr.row], D, resid, dangl);
         }
     } else {
-        const uint4 it = a.tiles[blockIdx.x - a.nlong];
+        const uint4 it = a.tiles[item - a.nlong];
         const uint32_t r0 = it.x, nrows = it.y - it.x, e0 = it.z;
         float val[kPerThread];
-        gather_tile(a, e0, it.w, val);
+        gather_tile<kHot>(a, e0, it.w, val, hot);
         // Row offsets (relative to the tile) and the rows' own degree and
         // old contribution, loaded while the gathers are in flight.
         for (uint32_t i = tid; i <= nrows; i += kBlock) roffs[i] = a.roff[r0 + i] - e0;
@@ -290,6 +310,8 @@ __global__ __launch_bounds__(kBlock) void pr_rows_kernel(IterArgs a)
             }
         }
     }
+    __syncthreads();  // the next item reuses the LDS staging
+    }
 
     resid = wave_sum_f64(resid);
     dangl = wave_sum_f64(dangl);
@@ -347,6 +369,8 @@ using namespace bw;
 struct bw_pr_plan {
     uint32_t V = 0, E = 0;
     uint32_t ntiles = 0, nlong = 0, npieces = 0;
+    uint32_t hot = 0;        // LDS hot-table size of the rows kernel (0, 8192, 16384)
+    uint32_t rows_grid = 0;  // rows-kernel workgroups (0: one per work item)
     uint4 *pieces = nullptr;
     LongRow *lrows = nullptr;
     uint4 *tiles = nullptr;
@@ -453,6 +477,16 @@ extern "C" int bw_pr_plan_info(const bw_pr_plan *plan, uint32_t *ntiles, uint32_
     return BW_HIP_OK;
 }
 
+extern "C" int bw_pr_plan_tune(bw_pr_plan *plan, uint32_t hot_vertices, uint32_t rows_grid)
+{
+    if (!plan) return fail(BW_HIP_ERR_ARG, "plan is NULL");
+    if (hot_vertices != 0 && hot_vertices != 8192 && hot_vertices != 16384)
+        return fail(BW_HIP_ERR_ARG, "hot_vertices must be 0, 8192 or 16384");
+    plan->hot = hot_vertices;
+    plan->rows_grid = rows_grid;
+    return BW_HIP_OK;
+}
+
 extern "C" int bw_pr_plan_degrees(const bw_pr_plan *plan, const uint32_t **deg)
 {
     if (!plan || !deg) return fail(BW_HIP_ERR_ARG, "invalid argument");
@@ -497,7 +531,11 @@ extern "C" int bw_pr_fused_run(const bw_graph_dev *g, const bw_pr_plan *plan, in
         BW_HIP_CHECK(hipEventCreate(&ev1));
         BW_HIP_CHECK(hipEventRecord(ev0, s));
     }
-    const uint32_t rowsGrid = plan->nlong + plan->ntiles;
+    const uint32_t items = plan->nlong + plan->ntiles;
+    const uint32_t rowsGrid = plan->rows_grid ? std::min(plan->rows_grid, items) : items;
+    auto rowsKernel = plan->hot == 16384 ? pr_rows_kernel<16384>
+                    : plan->hot == 8192  ? pr_rows_kernel<8192>
+                                         : pr_rows_kernel<0>;
     uint32_t nprev = initGrid;
     for (int32_t it = 1; it <= iterations; it++) {
         IterArgs a;
@@ -513,6 +551,7 @@ extern "C" int bw_pr_fused_run(const bw_graph_dev *g, const bw_pr_plan *plan, in
         a.partials = plan->partials;
         a.npieces = plan->npieces;
         a.nlong = plan->nlong;
+        a.ntiles = plan->ntiles;
         a.prev = plan->part;
         a.nprev = nprev;
         a.resid_out = it > 1 ? resid + (it - 2) : spare;
@@ -525,7 +564,7 @@ extern "C" int bw_pr_fused_run(const bw_graph_dev *g, const bw_pr_plan *plan, in
         hipLaunchKernelGGL(pr_pieces_kernel, dim3(plan->npieces + 1), dim3(kBlock), 0, s, a);
         BW_LAUNCH_CHECK();
         if (rowsGrid) {
-            hipLaunchKernelGGL(pr_rows_kernel, dim3(rowsGrid), dim3(kBlock), 0, s, a);
+            hipLaunchKernelGGL(rowsKernel, dim3(rowsGrid), dim3(kBlock), 0, s, a);
             BW_LAUNCH_CHECK();
         }
         nprev = rowsGrid;

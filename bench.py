@@ -177,6 +177,8 @@ def main() -> int:
     t0 = time.perf_counter()
     fused = pagerank.FusedPageRank.for_graph(g, max_iterations=ITERS, reorder=True)
     log(f"degree-ordered upload + plan in {time.perf_counter() - t0:.1f}s: {fused.info()}")
+    shapes = fused.autotune()
+    log(f"launch shapes (ms per 10 iterations): {shapes}; using hot/grid {fused.tuning}")
 
     # Warm-up, then K timed steps bracketed by device synchronisation.
     for _ in range(args.warmup):
@@ -222,6 +224,8 @@ def main() -> int:
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profile(),
                      "kernel": "pr_pieces_kernel + pr_rows_kernel (one iteration)",
+                     "launch_shape": {"hot_vertices": fused.tuning[0], "rows_grid": fused.tuning[1],
+                                      "autotune_ms_per_10_iterations": shapes},
                      "algorithmic_bytes_per_launch": bytes_per_launch,
                      "avg_launch_us": round(avg_launch_s * 1e6, 2)},
         "device": {"arch": props["arch"], "compute_units": props["compute_units"]},
