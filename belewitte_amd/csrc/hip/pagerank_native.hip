@@ -186,11 +186,15 @@ __device__ __forceinline__ void reduce_partials(const double2 *p, uint32_t n, do
     }
 }
 
+// The last workgroup reduces the previous launch's partials; the others
+// take pieces b, b + (gridDim.x - 1), ...
+template<uint32_t kHot>
 __global__ __launch_bounds__(kBlock) void pr_pieces_kernel(IterArgs a)
 {
     __shared__ float wsum[kBlock / kWave][2];
     __shared__ double red[kBlock / kWave][2];
-    if (blockIdx.x == a.npieces) {
+    __shared__ float hot[kHot > 0 ? kHot : 1];
+    if (blockIdx.x == gridDim.x - 1) {
         double r, g;
         reduce_partials(a.prev, a.nprev, r, g, red);
         if (threadIdx.x == 0) {
@@ -199,15 +203,23 @@ __global__ __launch_bounds__(kBlock) void pr_pieces_kernel(IterArgs a)
         }
         return;
     }
-    const uint4 pc = a.pieces[blockIdx.x];
-    float val[kPerThread];
-    gather_tile<0>(a, pc.z, pc.w, val, nullptr);
-    Kahan k;
+    if (kHot > 0) {
+        const uint32_t n = min(kHot, a.V);
+        for (uint32_t i = threadIdx.x; i < n; i += kBlock) hot[i] = a.cin[i];
+        __syncthreads();
+    }
+    for (uint32_t p = blockIdx.x; p < a.npieces; p += gridDim.x - 1) {
+        const uint4 pc = a.pieces[p];
+        float val[kPerThread];
+        gather_tile<kHot>(a, pc.z, pc.w, val, hot);
+        Kahan k;
 #pragma unroll
-    for (int q = 0; q < kPerThread; q++) k.add(val[q]);
-    float s = k.s, c = k.c;
-    block_kahan(s, c, wsum);
-    if (threadIdx.x == 0) a.partials[pc.y] = make_float2(s, c);
+        for (int q = 0; q < kPerThread; q++) k.add(val[q]);
+        float s = k.s, c = k.c;
+        block_kahan(s, c, wsum);
+        if (threadIdx.x == 0) a.partials[pc.y] = make_float2(s, c);
+        __syncthreads();  // wsum is reused by the next piece
+    }
 }
 
 // Work items: nlong long-row finalisers, then the row tiles.  Workgroup b
@@ -536,6 +548,12 @@ extern "C" int bw_pr_fused_run(const bw_graph_dev *g, const bw_pr_plan *plan, in
     auto rowsKernel = plan->hot == 16384 ? pr_rows_kernel<16384>
                     : plan->hot == 8192  ? pr_rows_kernel<8192>
                                          : pr_rows_kernel<0>;
+    auto piecesKernel = plan->hot == 16384 ? pr_pieces_kernel<16384>
+                      : plan->hot == 8192  ? pr_pieces_kernel<8192>
+                                           : pr_pieces_kernel<0>;
+    // Pieces: persistent like the rows when a grid is set, plus the
+    // reducing workgroup.
+    const uint32_t piecesGrid = (plan->rows_grid ? std::min(plan->rows_grid, plan->npieces) : plan->npieces) + 1;
     uint32_t nprev = initGrid;
     for (int32_t it = 1; it <= iterations; it++) {
         IterArgs a;
@@ -561,7 +579,7 @@ extern "C" int bw_pr_fused_run(const bw_graph_dev *g, const bw_pr_plan *plan, in
         a.mode = mode;
         a.not_last = it < iterations;
         a.cin_divided = (it > 1) || divide;
-        hipLaunchKernelGGL(pr_pieces_kernel, dim3(plan->npieces + 1), dim3(kBlock), 0, s, a);
+        hipLaunchKernelGGL(piecesKernel, dim3(piecesGrid), dim3(kBlock), 0, s, a);
         BW_LAUNCH_CHECK();
         if (rowsGrid) {
             hipLaunchKernelGGL(rowsKernel, dim3(rowsGrid), dim3(kBlock), 0, s, a);
