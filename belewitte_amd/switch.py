@@ -170,14 +170,17 @@ class Model:
         return cls(np.asarray(d["weights"], dtype=np.float32), int(d["outputs"]))
 
 
-def collect(engine: BFSEngine, sources, repeats: int = 1):
+def collect(engine: BFSEngine, sources, repeats: int = 1, progress=None):
     """Exhaustive per-level timings of every variant from each source.
-    Returns (Samples, {source: (levels, counts)})."""
+    Returns (Samples, {source: (levels, counts)}); `progress(i, source,
+    levels)` is called after each source."""
     records, results = [], {}
-    for s in sources:
+    for i, s in enumerate(sources):
         levels, counts, recs = engine.exhaustive(int(s), repeats=repeats)
         records.extend(recs)
         results[int(s)] = (levels, counts)
+        if progress:
+            progress(i, int(s), len(recs))
     return Samples.from_records(records), results
 
 
@@ -302,10 +305,12 @@ def main(argv=None) -> int:
     samples = None
     for g in training_graphs(args.small):
         t0 = time.perf_counter()
+        print(f"[switch] {g.name} generated", file=sys.stderr, flush=True)
         dg = device.DeviceGraph(g)
         eng = BFSEngine(dg)
         src = G.random_sources(g, args.sources, seed=5)
-        s, _ = collect(eng, src)
+        s, _ = collect(eng, src, progress=lambda i, v, n: print(
+            f"[switch] {g.name}: source {i + 1}/{len(src)} ({v}): {n} levels", file=sys.stderr, flush=True))
         print(f"[switch] {g.name}: {s.X.shape[0]} levels from {len(src)} sources in "
               f"{time.perf_counter() - t0:.1f}s; best-fixed totals {s.ms.sum(axis=0).round(3).tolist()} ms, "
               f"oracle {s.ms.min(axis=1).sum():.3f} ms", file=sys.stderr, flush=True)
