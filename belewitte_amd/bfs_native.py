@@ -21,7 +21,7 @@ from .device import DeviceGraph
 VARIANTS = ("td", "bu", "bu-wave")
 POLICIES = {"fixed": 0, "schedule": 1, "learned": 2, "host": 3}
 NUM_FEATURES = 8
-FEATURE_NAMES = ("log2_frontier", "log2_frontier_edges", "log2_unvisited", "depth",
+FEATURE_NAMES = ("log2_frontier_fraction", "log2_frontier_edge_fraction", "log2_unvisited_fraction", "depth",
                  "log2_mean_degree", "log2_std_degree", "prev_variant", "log2_prev_time_ns")
 UNVISITED = 2147483647
 DEFAULT_HUB_DEGREE = 4096

@@ -16,9 +16,10 @@
 //            separately and their edges are sliced across all workgroups.
 //   BU       bottom-up: one thread per unvisited vertex probes its
 //            in-neighbours and stops at the first one in the frontier.
-//   BU_WAVE  bottom-up with wave64 probing (64 in-neighbours per step,
-//            ballot early exit) for in-degree above the wave threshold
-//            (tunable, 32 by default), one thread per vertex below that.
+//   BU_WAVE  bottom-up where each lane probes only its vertex's first
+//            wave-threshold in-edges (tunable, 32 by default) and the whole
+//            wave finishes the rows still unresolved (256 in-edges per step,
+//            ballot early exit), so one long row never serialises a lane.
 // A search may start from several roots at once (all at depth 0), which
 // runs many small graphs stored as one disjoint union in one search.
 // Frontier membership for the bottom-up probes is a bitmap (V bits, three
@@ -211,13 +212,16 @@ __device__ void level_td(const Graph &g, const Buffers &b, uint32_t d, Emitter &
         }
     }
 
-    // Normal frontier: batches of 256 queue vertices, edges expanded 256 at
-    // a time with the owner found by binary search in the LDS degree scan.
+    // Normal frontier: batches of up to 256 queue vertices, edges expanded
+    // 256 at a time with the owner found by binary search in the LDS degree
+    // scan.  A small frontier is spread over all workgroups (batches down to
+    // one vertex), so its edges never pile up on a few workgroups.
     const uint32_t n = st.qcount;
-    for (uint32_t base = blockIdx.x * kBlock; base < n; base += gridDim.x * kBlock) {
+    const uint32_t bs = max(1u, min(static_cast<uint32_t>(kBlock), (n + gridDim.x - 1) / gridDim.x));
+    for (uint32_t base = blockIdx.x * bs; base < n; base += gridDim.x * bs) {
         const uint32_t i = base + tid;
         uint32_t deg = 0, start = 0;
-        if (i < n) {
+        if (tid < bs && i < n) {
             const uint32_t u = b.q[cur][i];
             start = g.off[u];
             deg = g.off[u + 1] - start;
@@ -270,26 +274,35 @@ __device__ void level_bu(const Graph &g, const Buffers &b, uint32_t d, Emitter &
             s = g.roff[v];
             e = g.roff[v + 1];
         }
-        const bool big = wave && e - s > b.wave_degree;
+        // Each lane first probes its own vertex's in-edges, four independent
+        // probes per step; BU_WAVE stops a lane after wave_degree edges and
+        // hands the rest of a still-unresolved row to the whole wave.
+        const uint32_t lim = wave && e - s > b.wave_degree ? s + b.wave_degree : e;
         bool discovered = false;
-        if (unvisited && !big) {
-            for (uint32_t j = s; j < e; j++) {
-                if (test_bit(front, g.rcol[j])) {
-                    discovered = true;
-                    break;
-                }
+        for (uint32_t j = s; j < lim; j += 4) {
+            bool hit = false;
+#pragma unroll
+            for (uint32_t q = 0; q < 4; q++)
+                if (j + q < lim) hit |= test_bit(front, g.rcol[j + q]);
+            if (hit) {
+                discovered = true;
+                break;
             }
         }
         if (wave) {
-            uint64_t m = __ballot(big);
+            uint64_t m = __ballot(!discovered && lim < e);
             while (m) {
                 const int l = __ffsll(static_cast<unsigned long long>(m)) - 1;
                 m &= m - 1;
-                const uint32_t vs = __shfl(s, l, kWave), ve = __shfl(e, l, kWave);
+                const uint32_t vs = __shfl(lim, l, kWave), ve = __shfl(e, l, kWave);
                 bool found = false;
-                for (uint32_t k0 = vs; k0 < ve && !found; k0 += kWave) {
-                    const uint32_t j = k0 + lane;
-                    const bool hit = j < ve && test_bit(front, g.rcol[j]);
+                for (uint32_t k0 = vs; k0 < ve && !found; k0 += 4 * kWave) {
+                    bool hit = false;
+#pragma unroll
+                    for (uint32_t q = 0; q < 4; q++) {
+                        const uint32_t j = k0 + q * kWave + lane;
+                        if (j < ve) hit |= test_bit(front, g.rcol[j]);
+                    }
                     found = __ballot(hit) != 0;
                 }
                 if (found && lane == l) discovered = true;
@@ -376,19 +389,21 @@ __global__ __launch_bounds__(kBlock) void bfs_engine_step(Graph g, Buffers b)
 __host__ __device__ constexpr int mlp_len(int K)
 { return 2 * kFeatures + kHidden * kFeatures + kHidden + kHidden * kHidden + kHidden + K * kHidden + K; }
 
-// The 8 features of the pending level: frontier size, frontier out-edges,
-// unvisited vertices, depth, mean and standard deviation of the frontier
-// out-degree, previous variant, previous level time (counts and times as
-// log2(1 + x)).
-__host__ __device__ void features(const BfsState &st, uint64_t V, float *f)
+// The 8 features of the pending level: frontier size, frontier out-edges and
+// unvisited vertices as log2 fractions of the graph (log2((1 + x) / V),
+// log2((1 + x) / E)), which carry over between graph scales; depth;
+// log2(1 + x) of the mean and standard deviation of the frontier
+// out-degree; previous variant; log2(1 + previous level time in ns).
+__host__ __device__ void features(const BfsState &st, uint64_t V, uint64_t E, float *f)
 {
     const double n = static_cast<double>(st.qcount) + static_cast<double>(st.hubcount);
     const double edges = static_cast<double>(st.frontier_edges);
     const double mean = n > 0 ? edges / n : 0.0;
     const double var = n > 0 ? fmax(st.frontier_sq / n - mean * mean, 0.0) : 0.0;
-    f[0] = static_cast<float>(log2(1.0 + n));
-    f[1] = static_cast<float>(log2(1.0 + edges));
-    f[2] = static_cast<float>(log2(1.0 + static_cast<double>(V - st.visited)));
+    const double lv = log2(static_cast<double>(V)), le = log2(static_cast<double>(E > 0 ? E : 1));
+    f[0] = static_cast<float>(log2(1.0 + n) - lv);
+    f[1] = static_cast<float>(log2(1.0 + edges) - le);
+    f[2] = static_cast<float>(log2(1.0 + static_cast<double>(V - st.visited)) - lv);
     f[3] = static_cast<float>(st.depth);
     f[4] = static_cast<float>(log2(1.0 + mean));
     f[5] = static_cast<float>(log2(1.0 + sqrt(var)));
@@ -436,7 +451,7 @@ __device__ int32_t choose(const PolicyArgs &p, const BfsState &st, const Graph &
     switch (p.policy) {
     case SCHEDULE: return p.schedule[st.depth % static_cast<uint32_t>(p.schedule_len)];
     case LEARNED: {
-        if (threadIdx.x == 0) features(st, g.V, f);
+        if (threadIdx.x == 0) features(st, g.V, g.E, f);
         __syncthreads();
         const int32_t c = mlp_choose(p.mlp, p.nout, f, h1, h2, out);
         return c < kVariants ? c : p.fixed;
@@ -603,6 +618,23 @@ __global__ __launch_bounds__(kBlock) void bfs_engine_init(Graph g, Buffers b, Po
 __global__ void set_choice(BfsState *st, int32_t c) { st->choice = c; }
 __global__ void copy_state(BfsState *dst, const BfsState *src) { *dst = *src; }
 
+// Bitmap of the vertices without in-edges: no search can reach them except
+// as a root, so each search starts with them marked visited and bottom-up
+// levels never scan them.  A wave covers 64 consecutive vertices, i.e. two
+// whole words.
+__global__ __launch_bounds__(kBlock) void no_in_edges_kernel(Graph g, uint32_t *bits)
+{
+    const int lane = __lane_id();
+    for (uint64_t v = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x;
+         v - lane < g.V; v += static_cast<uint64_t>(gridDim.x) * kBlock) {
+        const bool none = v < g.V && g.roff[v] == g.roff[v + 1];
+        const uint64_t m = __ballot(none);
+        const uint64_t w = (v - lane) >> 5;
+        if (lane == 0) bits[w] = static_cast<uint32_t>(m);
+        if (lane == 32 && ((v - lane) + 32) < g.V) bits[w + 1] = static_cast<uint32_t>(m >> 32);
+    }
+}
+
 }  // namespace
 }  // namespace bw
 
@@ -625,6 +657,7 @@ struct bw_bfs_engine {
     int32_t schedule_cap = 0;
     uint32_t *roots = nullptr;
     uint32_t roots_cap = 0;
+    uint32_t *no_in = nullptr;  // vertices without in-edges (bitmap)
     hipStream_t own_stream = nullptr;
     hipGraphExec_t exec = nullptr;
     int32_t exec_key[4] = {-1, -1, -1, -1};  // policy, fixed, schedule_len, levels per batch
@@ -641,7 +674,7 @@ struct bw_bfs_engine {
         drop_exec();
         void *bufs[] = {own_levels, b.q[0], b.q[1], b.hubq[0], b.hubq[1], b.hub_prefix, b.bits[0], b.bits[1],
                         b.bits[2], b.vis, b.st, b.part, b.counts, b.times, b.choices, snap_levels, snap_q, snap_hub,
-                        snap_bits, snap_prefix, snap_st, mlp, schedule, roots};
+                        snap_bits, snap_prefix, snap_st, mlp, schedule, roots, no_in};
         for (void *p : bufs)
             if (p) (void)hipFree(p);
         if (own_stream) (void)hipStreamDestroy(own_stream);
@@ -723,6 +756,7 @@ extern "C" int bw_bfs_engine_create(bw_bfs_engine **out, const bw_graph_dev *g, 
     alloc(&e->b.hub_prefix, sizeof(uint64_t) * (kHubCap + 1));
     for (auto &bm : e->b.bits) alloc(&bm, sizeof(uint32_t) * words);
     alloc(&e->b.vis, sizeof(uint32_t) * words);
+    alloc(&e->no_in, sizeof(uint32_t) * words);
     alloc(&e->b.st, sizeof(BfsState));
     alloc(&e->b.part, sizeof(uint64_t) * 2 * e->grid);
     alloc(&e->b.counts, sizeof(uint32_t) * max_levels);
@@ -736,6 +770,11 @@ extern "C" int bw_bfs_engine_create(bw_bfs_engine **out, const bw_graph_dev *g, 
     if (!ok) {
         delete e;
         return fail(BW_HIP_ERR_NOMEM, "device allocation of the BFS engine failed");
+    }
+    hipLaunchKernelGGL(no_in_edges_kernel, dim3(e->grid), dim3(kBlock), 0, nullptr, e->g, e->no_in);
+    if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess) {
+        delete e;
+        return fail(BW_HIP_ERR_RUNTIME, "building the in-edge bitmap failed");
     }
     *out = e;
     return BW_HIP_OK;
@@ -802,7 +841,7 @@ int begin_impl(bw_bfs_engine *e, uint32_t root, const uint32_t *roots_dev, uint3
     BW_TRY(check_policy(e, policy, fixed, schedule_len));
     BW_TRY(bw_hip_fill_u32(e->b.levels, static_cast<uint32_t>(kUnvisited), e->g.V, s));
     for (uint32_t *bm : e->b.bits) BW_TRY(bw_hip_fill_u32(bm, 0u, e->b.words, s));
-    BW_TRY(bw_hip_fill_u32(e->b.vis, 0u, e->b.words, s));
+    BW_HIP_CHECK(hipMemcpyAsync(e->b.vis, e->no_in, sizeof(uint32_t) * e->b.words, hipMemcpyDeviceToDevice, s));
     hipLaunchKernelGGL(bfs_engine_init, dim3(1), dim3(kBlock), 0, s, e->g, e->b,
                        make_policy(e, policy, fixed, schedule_len), root, roots_dev, nroots);
     BW_LAUNCH_CHECK();
@@ -884,7 +923,7 @@ extern "C" int bw_bfs_engine_status(bw_bfs_engine *e, uint32_t *done, uint32_t *
     if (depth) *depth = st.depth;
     if (num_levels) *num_levels = st.num_levels;
     if (choice) *choice = st.choice;
-    if (feat) features(st, e->g.V, feat);
+    if (feat) features(st, e->g.V, e->g.E, feat);
     return BW_HIP_OK;
 }
 

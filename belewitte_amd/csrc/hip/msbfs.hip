@@ -76,20 +76,20 @@ __global__ __launch_bounds__(kBlock) void msbfs_level(MsArgs a)
             s = a.roff[v];
             e = a.roff[v + 1];
         }
-        const bool big = open && e - s > a.wave_degree;
+        // The lane probes the first wave_degree in-edges itself; a row still
+        // missing searches after that is finished by the whole wave.
+        const uint32_t lim = e - s > a.wave_degree ? s + a.wave_degree : e;
         uint64_t acc = 0;
-        if (open && !big) {
-            for (uint32_t j = s; j < e; j++) {
-                acc |= fin[a.rcol[j]];
-                if ((acc | vis) == a.all) break;
-            }
+        for (uint32_t j = s; j < lim; j++) {
+            acc |= fin[a.rcol[j]];
+            if ((acc | vis) == a.all) break;
         }
-        uint64_t m = __ballot(big);
+        uint64_t m = __ballot(open && lim < e && (acc | vis) != a.all);
         while (m) {
             const int l = __ffsll(static_cast<unsigned long long>(m)) - 1;
             m &= m - 1;
-            const uint32_t vs = __shfl(s, l, kWave), ve = __shfl(e, l, kWave);
-            const uint64_t lvis = __shfl(vis, l, kWave);
+            const uint32_t vs = __shfl(lim, l, kWave), ve = __shfl(e, l, kWave);
+            const uint64_t lvis = __shfl(vis | acc, l, kWave);
             uint64_t wacc = 0;
             for (uint32_t k0 = vs; k0 < ve; k0 += kWave) {
                 const uint32_t j = k0 + lane;
@@ -98,7 +98,7 @@ __global__ __launch_bounds__(kBlock) void msbfs_level(MsArgs a)
                 wacc |= x;
                 if ((wacc | lvis) == a.all) break;
             }
-            if (lane == l) acc = wacc;
+            if (lane == l) acc |= wacc;
         }
         const uint64_t nb = acc & ~vis;
         if (live) {

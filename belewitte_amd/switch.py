@@ -184,12 +184,44 @@ def collect(engine: BFSEngine, sources, repeats: int = 1, progress=None):
     return Samples.from_records(records), results
 
 
-def fit(samples: Samples, epochs: int = 2000, batch: int = 64, lr: float = 3e-3, seed: int = 0):
+def fit(samples: Samples, epochs: int = 100, batch: int = 64, lr: float = 3e-3, seed: int = 0):
     """A model trained on the GPU on `samples`; returns (model, epoch losses)."""
     Y = samples.targets
     m = Model.init(samples.X, seed=seed, target_mean=float(Y.mean()) if Y.size else 0.0)
     loss = m.train(samples.X, Y, epochs=epochs, batch=batch, lr=lr, seed=seed + 1)
     return m, loss
+
+
+def balanced(groups) -> Samples:
+    """The groups concatenated, each repeated so that every group weighs
+    about alike in the loss."""
+    most = max(g.X.shape[0] for g in groups)
+    out = None
+    for g in groups:
+        for _ in range(max(1, int(round(most / g.X.shape[0])))):
+            out = g if out is None else out.concat(g)
+    return out
+
+
+def regret(samples: Samples, choices: np.ndarray) -> float:
+    """Total time of the per-level choices over the per-level optimum, - 1."""
+    got = samples.ms[np.arange(samples.ms.shape[0]), choices].sum()
+    return float(got / samples.ms.min(axis=1).sum() - 1.0)
+
+
+def cross_validate(groups, candidates=(25, 50, 100, 200, 400, 800), seed: int = 0) -> dict:
+    """Leave-one-graph-out regret of the model trained for each epoch count
+    in `candidates` (mean over held-out graphs): longer training fits the
+    training graphs ever closer and carries over to other graphs worse."""
+    table = {}
+    for ep in candidates:
+        r = []
+        for i, held in enumerate(groups):
+            rest = [g for j, g in enumerate(groups) if j != i]
+            m, _ = fit(balanced(rest), epochs=ep, seed=seed)
+            r.append(regret(held, m.choose(held.X)))
+        table[ep] = float(np.mean(r))
+    return table
 
 
 def simulate(samples: Samples, choices: np.ndarray) -> dict:
@@ -206,7 +238,7 @@ def simulate(samples: Samples, choices: np.ndarray) -> dict:
 
 
 def evaluate(engine: BFSEngine, g, sources, model: Model, repeats: int = 1, runs: int = 3,
-             levels_per_sync: int = 8) -> dict:
+             levels_per_sync: int = 8, samples_out=None) -> dict:
     """Learned switch vs best fixed vs per-level oracle on `sources`.
 
     Simulated: sums of exhaustive per-level timings (device time of each
@@ -214,6 +246,8 @@ def evaluate(engine: BFSEngine, g, sources, model: Model, repeats: int = 1, runs
     level loop on the device (hipGraph batches), min over `runs`; the
     learned policy evaluates the model on the device at every level."""
     samples, results = collect(engine, sources, repeats=repeats)
+    if samples_out:
+        samples.save(samples_out)
     learned_choice = model.choose(samples.X)
     sim = simulate(samples, learned_choice)
     K = len(VARIANTS)
@@ -234,8 +268,10 @@ def evaluate(engine: BFSEngine, g, sources, model: Model, repeats: int = 1, runs
         levels_ref, counts_ref = results[int(s)]
         edges += teps_edges(g, levels_ref)
         for k in range(K):
+            # A variant far off the best in the simulation is run once only.
+            reps = runs if fixed_tot[k] <= 2.0 * fixed_tot[best_fixed] else 1
             best = np.inf
-            for _ in range(runs):
+            for _ in range(reps):
                 r = engine.run(int(s), "fixed", k, levels_per_sync=levels_per_sync, download=False)
                 best = min(best, r.total_ms)
             exec_fixed[k] += best
@@ -296,13 +332,13 @@ def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description="collect exhaustive per-level BFS timings and train the switch")
     ap.add_argument("--out", default=str(MODEL_PATH))
     ap.add_argument("--samples-out", default=None)
-    ap.add_argument("--sources", type=int, default=16)
-    ap.add_argument("--epochs", type=int, default=3000)
+    ap.add_argument("--sources", type=int, default=48)
+    ap.add_argument("--epochs", type=int, default=0, help="training epochs (0: chosen by cross-validation)")
     ap.add_argument("--small", action="store_true")
     args = ap.parse_args(argv)
     device.require_gpu()
     device.set_device(0)
-    samples = None
+    groups = []
     for g in training_graphs(args.small):
         t0 = time.perf_counter()
         print(f"[switch] {g.name} generated", file=sys.stderr, flush=True)
@@ -314,12 +350,22 @@ def main(argv=None) -> int:
         print(f"[switch] {g.name}: {s.X.shape[0]} levels from {len(src)} sources in "
               f"{time.perf_counter() - t0:.1f}s; best-fixed totals {s.ms.sum(axis=0).round(3).tolist()} ms, "
               f"oracle {s.ms.min(axis=1).sum():.3f} ms", file=sys.stderr, flush=True)
-        samples = s if samples is None else samples.concat(s)
+        s.source = s.source + (np.int64(len(groups)) << 32)  # sources unique across graphs
+        groups.append(s)
         eng.close()
         dg.free()
+    samples = groups[0]
+    for s in groups[1:]:
+        samples = samples.concat(s)
     if args.samples_out:
         samples.save(args.samples_out)
-    model, loss = fit(samples, epochs=args.epochs)
+    epochs = args.epochs
+    if epochs <= 0:
+        cv = cross_validate(groups)
+        epochs = min(cv, key=cv.get)
+        print(f"[switch] leave-one-graph-out regret by epochs: "
+              f"{ {k: round(v, 4) for k, v in cv.items()} }; training {epochs} epochs", file=sys.stderr, flush=True)
+    model, loss = fit(balanced(groups), epochs=epochs)
     choice = model.choose(samples.X)
     sim = simulate(samples, choice)
     fixed = np.sum([v["fixed"] for v in sim.values()], axis=0)

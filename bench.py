@@ -62,7 +62,7 @@ def cpu_baseline(g, ranks_gpu, threads: int):
                       f"(uniform dangling) over all {g.edge_count} edges, {dt:.2f} s"}, err, csr
 
 
-def bfs_learned_switch(scale: int, sources: int) -> dict:
+def bfs_learned_switch(scale: int, sources: int, samples_out=None) -> dict:
     """Undirected R-MAT (scale, edge factor 16, seed 5), `sources` random
     sources of out-degree >= 1: the learned per-level switch against the
     best fixed native variant and the per-level oracle, simulated on
@@ -81,7 +81,7 @@ def bfs_learned_switch(scale: int, sources: int) -> dict:
         model, _ = switch.fit(samples)
         origin = "trained in this run on 4 of the sources (no stored model)"
     t0 = time.perf_counter()
-    res = switch.evaluate(eng, g, src, model, repeats=2, runs=3)
+    res = switch.evaluate(eng, g, src, model, repeats=2, runs=3, samples_out=samples_out)
     log(f"bfs switch evaluated in {time.perf_counter() - t0:.1f}s: {res['executed_ms']}")
     # Algorithmic bytes of a top-down pass per source: 8 E_r + 16 V_r.
     deg = np.diff(g.offsets()).astype(np.int64)
@@ -159,6 +159,7 @@ def main() -> int:
     ap.add_argument("--no-bfs", action="store_true", help="skip the BFS figures")
     ap.add_argument("--bfs-scale", type=int, default=25, help="R-MAT scale of the learned-switch BFS")
     ap.add_argument("--bfs-sources", type=int, default=16)
+    ap.add_argument("--bfs-samples-out", default=None, help="save the exhaustive per-level timings (.npz)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
@@ -270,7 +271,7 @@ def main() -> int:
     dg.free()
 
     if not args.no_bfs:
-        out["bfs_learned_switch"] = bfs_learned_switch(args.bfs_scale, args.bfs_sources)
+        out["bfs_learned_switch"] = bfs_learned_switch(args.bfs_scale, args.bfs_sources, args.bfs_samples_out)
         out["bfs_batched"] = bfs_batched()
 
     print(json.dumps(out), flush=True)

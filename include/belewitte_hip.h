@@ -433,10 +433,12 @@ typedef struct bw_bfs_engine bw_bfs_engine;
 #define BW_POLICY_LEARNED 2
 #define BW_POLICY_HOST 3
 
-/* Features of a level, reduced on the device: log2(1 + x) of frontier
- * vertices, frontier out-edges and unvisited vertices; depth; log2(1 + x)
- * of the mean and standard deviation of the frontier out-degree; previous
- * variant (-1 at level 0); log2(1 + previous level time in ns). */
+/* Features of a level, reduced on the device: frontier vertices, frontier
+ * out-edges and unvisited vertices as log2 fractions of the graph
+ * (log2((1 + x) / V), log2((1 + x) / E), log2((1 + x) / V)); depth;
+ * log2(1 + x) of the mean and standard deviation of the frontier
+ * out-degree; previous variant (-1 at level 0); log2(1 + previous level
+ * time in ns). */
 #define BW_BFS_FEATURES 8
 
 /* levels: device int32[V] receiving the depths, or NULL for an

@@ -141,8 +141,9 @@ def test_device_features_match_host(cases):
             n = fr.size
             mean = fr.mean() if n else 0.0
             std = np.sqrt(max((fr ** 2).mean() - mean * mean, 0.0)) if n else 0.0
-            want = [np.log2(1 + n), np.log2(1 + fr.sum()), np.log2(1 + V - np.count_nonzero(levels <= d)),
-                    d, np.log2(1 + mean), np.log2(1 + std)]
+            lv, le = np.log2(V), np.log2(max(g.edge_count, 1))
+            want = [np.log2(1 + n) - lv, np.log2(1 + fr.sum()) - le,
+                    np.log2(1 + V - np.count_nonzero(levels <= d)) - lv, d, np.log2(1 + mean), np.log2(1 + std)]
             assert np.allclose(f[:6], want, rtol=1e-5, atol=1e-5), (g.name, d, f, want)
             assert f[6] == (-1 if d == 0 else (d - 1) % 3)
             assert (f[7] == 0) == (d == 0)
