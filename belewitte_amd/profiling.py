@@ -4,8 +4,11 @@
         One streaming copy of the given size (bw_hip_copy_kernel): reads and
         writes exactly that many bytes, the reference point that scales the
         FETCH_SIZE / WRITE_SIZE counters to bytes.
-    python -m belewitte_amd.profiling pagerank --scale 22 --iterations 5
+    python -m belewitte_amd.profiling pagerank --scale 22 --iterations 5 [--reorder] [--grid N --hot H]
         A few fused PageRank iterations on the benchmark graph.
+    python -m belewitte_amd.profiling bfs --scale 25 --sources 4 [--policy learned|td|bu|bu-wave]
+        BFS engine searches on the undirected R-MAT of the learned-switch
+        workload (the shipped model for the learned policy).
 
 Run them under `rocprofv3 --pmc ... --kernel-trace -- python3 -m ...`.
 """
@@ -32,7 +35,7 @@ def copy(mib: int, repeat: int) -> None:
     print(f"copy: {repeat} x {n * 4} bytes", flush=True)
 
 
-def pagerank(scale: int, iterations: int, reorder: bool) -> None:
+def pagerank(scale: int, iterations: int, reorder: bool, hot: int = 0, grid: int = 0) -> None:
     from . import device, graph, pagerank as pr
     device.require_gpu()
     device.set_device(0)
@@ -41,8 +44,27 @@ def pagerank(scale: int, iterations: int, reorder: bool) -> None:
         g, _ = g.reorder_by_degree()
     dg = device.DeviceGraph(g, reps=("rev_csr",))
     f = pr.FusedPageRank(dg, max_iterations=iterations)
+    f.tune(hot, grid)
     _, _, ms = f.run(iterations, "uniform", download=False)
     print(f"pagerank: V={g.vertex_count} E={g.edge_count} {iterations} iterations {ms:.3f} ms", flush=True)
+
+
+def bfs(scale: int, sources: int, policy: str) -> None:
+    from . import bfs_native, device, graph, switch
+    device.require_gpu()
+    device.set_device(0)
+    g = graph.rmat(scale, 16, seed=5, undirected=True)
+    dg = device.DeviceGraph(g)
+    eng = bfs_native.BFSEngine(dg)
+    if policy == "learned":
+        eng.set_model(switch.Model.load(switch.MODEL_PATH).w)
+    for s in graph.random_sources(g, sources, seed=8):
+        if policy == "learned":
+            r = eng.run(s, "learned", "td", download=False)
+        else:
+            r = eng.run(s, "fixed", policy, download=False)
+        print(f"bfs {policy}: source {s} {r.num_levels} levels {r.total_ms:.3f} ms "
+              f"variants {r.choices.tolist()}", flush=True)
 
 
 def main(argv=None) -> int:
@@ -55,11 +77,19 @@ def main(argv=None) -> int:
     p.add_argument("--scale", type=int, default=22)
     p.add_argument("--iterations", type=int, default=5)
     p.add_argument("--reorder", action="store_true")
+    p.add_argument("--hot", type=int, default=0)
+    p.add_argument("--grid", type=int, default=0)
+    b = sub.add_parser("bfs")
+    b.add_argument("--scale", type=int, default=25)
+    b.add_argument("--sources", type=int, default=4)
+    b.add_argument("--policy", default="learned")
     a = ap.parse_args(argv)
     if a.cmd == "copy":
         copy(a.mib, a.repeat)
+    elif a.cmd == "pagerank":
+        pagerank(a.scale, a.iterations, a.reorder, a.hot, a.grid)
     else:
-        pagerank(a.scale, a.iterations, a.reorder)
+        bfs(a.scale, a.sources, a.policy)
     return 0
 
 
