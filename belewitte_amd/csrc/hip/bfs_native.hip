@@ -412,7 +412,8 @@ __host__ __device__ void features(const BfsState &st, uint64_t V, uint64_t E, fl
 }
 
 // One workgroup: MLP forward on the features (one output per variant, the
-// predicted log2 time), returns the argmin.
+// predicted log2 time), returns the argmin.  `w` is the workgroup's LDS
+// copy of the packed weights (see stage_model).
 __device__ int32_t mlp_choose(const float *w, int K, const float *f, float *h1, float *h2, float *out)
 {
     const float *mean = w, *sd = w + kFeatures;
@@ -444,16 +445,26 @@ __device__ int32_t mlp_choose(const float *w, int K, const float *f, float *h1, 
     return best;
 }
 
-// Block-uniform; the result is valid in every thread.
-__device__ int32_t choose(const PolicyArgs &p, const BfsState &st, const Graph &g, float *f, float *h1, float *h2,
-                          float *out)
+// Copies the packed weights into LDS; issued first in a kernel so the loads
+// overlap the work before the first barrier.
+__device__ __forceinline__ void stage_model(const PolicyArgs &p, float *sw)
+{
+    if (p.policy != LEARNED) return;
+    const int n = mlp_len(p.nout);
+    for (int i = threadIdx.x; i < n; i += kBlock) sw[i] = p.mlp[i];
+}
+
+// Block-uniform; the result is valid in every thread.  sw holds the staged
+// weights (stage_model, followed by at least one barrier).
+__device__ int32_t choose(const PolicyArgs &p, const BfsState &st, const Graph &g, const float *sw, float *f,
+                          float *h1, float *h2, float *out)
 {
     switch (p.policy) {
     case SCHEDULE: return p.schedule[st.depth % static_cast<uint32_t>(p.schedule_len)];
     case LEARNED: {
         if (threadIdx.x == 0) features(st, g.V, g.E, f);
         __syncthreads();
-        const int32_t c = mlp_choose(p.mlp, p.nout, f, h1, h2, out);
+        const int32_t c = mlp_choose(sw, p.nout, f, h1, h2, out);
         return c < kVariants ? c : p.fixed;
     }
     case HOST: return st.choice;
@@ -469,10 +480,12 @@ __global__ __launch_bounds__(kBlock) void bfs_engine_epilogue(Graph g, Buffers b
     __shared__ double rsq[kBlock / kWave];
     __shared__ uint64_t hscan[kBlock];
     __shared__ float f[kFeatures], h1[kHidden], h2[kHidden], out[kMaxOut];
+    __shared__ float sw[mlp_len(kMaxOut)];
 
     BfsState &st = *b.st;
     if (st.done) return;
     const uint64_t now = wall_clock64();
+    stage_model(p, sw);
     const int tid = threadIdx.x, lane = tid & (kWave - 1), wid = tid / kWave;
     uint64_t os = 0;
     double sq = 0.0;
@@ -558,7 +571,7 @@ __global__ __launch_bounds__(kBlock) void bfs_engine_epilogue(Graph g, Buffers b
     }
     __syncthreads();
     if (!st.done) {
-        const int32_t c = choose(p, st, g, f, h1, h2, out);
+        const int32_t c = choose(p, st, g, sw, f, h1, h2, out);
         __syncthreads();
         if (tid == 0) st.choice = c;
     }
@@ -570,7 +583,9 @@ __global__ __launch_bounds__(kBlock) void bfs_engine_init(Graph g, Buffers b, Po
                                                           const uint32_t *roots, uint32_t nroots)
 {
     __shared__ float f[kFeatures], h1[kHidden], h2[kHidden], out[kMaxOut];
+    __shared__ float sw[mlp_len(kMaxOut)];
     BfsState &st = *b.st;
+    stage_model(p, sw);
     if (threadIdx.x == 0) {
         uint32_t nq = 0, nh = 0;
         uint64_t visited = 0, edges = 0;
@@ -610,7 +625,7 @@ __global__ __launch_bounds__(kBlock) void bfs_engine_init(Graph g, Buffers b, Po
         st.choice = p.fixed;
     }
     __syncthreads();
-    const int32_t c = choose(p, st, g, f, h1, h2, out);
+    const int32_t c = choose(p, st, g, sw, f, h1, h2, out);
     __syncthreads();
     if (threadIdx.x == 0) st.choice = c;
 }

@@ -120,14 +120,15 @@ __device__ __forceinline__ void finalise(const IterArgs &a, uint32_t v, float su
 // With kHot > 0, sources below kHot are read from the workgroup's LDS copy
 // of their contributions (`hot`) instead of through the L2: on a graph
 // relabelled by decreasing out-degree those are the most gathered sources.
+// `t` is the thread's index within its group of kBlock threads.
 template<uint32_t kHot>
 __device__ __forceinline__ void gather_tile(const IterArgs &a, uint32_t e, uint32_t e_end, float (&val)[kPerThread],
-                                            const float *hot)
+                                            const float *hot, uint32_t t)
 {
     uint32_t col[kPerThread];
 #pragma unroll
     for (int k = 0; k < kPerThread; k++) {
-        const uint32_t j = e + threadIdx.x + kBlock * k;
+        const uint32_t j = e + t + kBlock * k;
         col[k] = j < e_end ? __builtin_nontemporal_load(&a.rcols[j]) : 0xFFFFFFFFu;
     }
 #pragma unroll
@@ -211,7 +212,7 @@ __global__ __launch_bounds__(kBlock) void pr_pieces_kernel(IterArgs a)
     for (uint32_t p = blockIdx.x; p < a.npieces; p += gridDim.x - 1) {
         const uint4 pc = a.pieces[p];
         float val[kPerThread];
-        gather_tile<kHot>(a, pc.z, pc.w, val, hot);
+        gather_tile<kHot>(a, pc.z, pc.w, val, hot, threadIdx.x);
         Kahan k;
 #pragma unroll
         for (int q = 0; q < kPerThread; q++) k.add(val[q]);
@@ -222,123 +223,143 @@ __global__ __launch_bounds__(kBlock) void pr_pieces_kernel(IterArgs a)
     }
 }
 
-// Work items: nlong long-row finalisers, then the row tiles.  Workgroup b
-// takes items b, b + gridDim.x, ... (one item each when the grid covers
-// them all; persistent otherwise, which also loads the kHot table once per
-// workgroup rather than once per tile).
-template<uint32_t kHot>
-__global__ __launch_bounds__(kBlock) void pr_rows_kernel(IterArgs a)
+// LDS staging of one row tile.
+struct TileLds {
+    float vals[kTileNnz];
+    uint32_t roffs[kRowsMax + 1];
+    uint32_t sdeg[kRowsMax];
+    float sold[kRowsMax];
+    uint32_t bigRows[kTileNnz / kBigRow];
+    uint32_t nbig;
+};
+
+// Work items: nlong long-row finalisers, then the row tiles.  A workgroup
+// holds kSub groups of kBlock threads that take kSub consecutive items per
+// step: workgroup b takes steps b, b + gridDim.x, ... (one step each when
+// the grid covers all items; persistent otherwise).  The kHot table is
+// loaded once per workgroup and shared by its groups.  Every group passes
+// the same barriers whatever its item (or none), so the block-wide
+// barriers stay matched.
+template<uint32_t kHot, int kSub>
+__global__ __launch_bounds__(kBlock * kSub) void pr_rows_kernel(IterArgs a)
 {
-    __shared__ float vals[kTileNnz];
-    __shared__ uint32_t roffs[kRowsMax + 1];
-    __shared__ uint32_t sdeg[kRowsMax];
-    __shared__ float sold[kRowsMax];
-    __shared__ uint32_t bigRows[kTileNnz / kBigRow];
-    __shared__ uint32_t nbig;
-    __shared__ double red[kBlock / kWave][2];
+    __shared__ TileLds T[kSub];
+    __shared__ double red[kBlock * kSub / kWave][2];
     __shared__ float hot[kHot > 0 ? kHot : 1];
 
-    const int tid = threadIdx.x;
+    const int g = threadIdx.x / kBlock;   // group
+    const int tid = threadIdx.x % kBlock;  // thread within the group
     const int lane = tid & (kWave - 1), wid = tid / kWave;
+    TileLds &L = T[g];
     const double D = (a.mode == BW_PR_DANGLING_UNIFORM) ? *a.dangl : 0.0;
     double resid = 0.0, dangl = 0.0;
     if (kHot > 0) {
         const uint32_t n = min(kHot, a.V);
-        for (uint32_t i = tid; i < n; i += kBlock) hot[i] = a.cin[i];
+        for (uint32_t i = threadIdx.x; i < n; i += kBlock * kSub) hot[i] = a.cin[i];
         __syncthreads();
     }
 
-    for (uint32_t item = blockIdx.x; item < a.nlong + a.ntiles; item += gridDim.x) {
-    if (item < a.nlong) {
-        if (tid == 0) {
-            const LongRow lr = a.lrows[item];
-            float s = 0.0f, c = 0.0f;
-            for (uint32_t q = 0; q < lr.pieces; q++) {
-                const float2 p = a.partials[lr.base + q];
-                kahan_merge(s, c, p.x, p.y);
+    const uint32_t nitems = a.nlong + a.ntiles;
+    for (uint32_t step = blockIdx.x; step * kSub < nitems; step += gridDim.x) {
+        const uint32_t item = step * kSub + g;
+        const bool isTile = item >= a.nlong && item < nitems;
+        uint32_t r0 = 0, nrows = 0;
+        if (item < a.nlong) {
+            if (tid == 0) {
+                const LongRow lr = a.lrows[item];
+                float s = 0.0f, c = 0.0f;
+                for (uint32_t q = 0; q < lr.pieces; q++) {
+                    const float2 p = a.partials[lr.base + q];
+                    kahan_merge(s, c, p.x, p.y);
+                }
+                finalise(a, lr.row, s - c, a.deg[lr.row], a.cin[lr.row], D, resid, dangl);
             }
-            finalise(a, lr.row, s - c, a.deg[lr.row], a.cin[lr.row], D, resid, dangl);
-        }
-    } else {
-        const uint4 it = a.tiles[item - a.nlong];
-        const uint32_t r0 = it.x, nrows = it.y - it.x, e0 = it.z;
-        float val[kPerThread];
-        gather_tile<kHot>(a, e0, it.w, val, hot);
-        // Row offsets (relative to the tile) and the rows' own degree and
-        // old contribution, loaded while the gathers are in flight.
-        for (uint32_t i = tid; i <= nrows; i += kBlock) roffs[i] = a.roff[r0 + i] - e0;
-        for (uint32_t i = tid; i < nrows; i += kBlock) {
-            sdeg[i] = a.deg[r0 + i];
-            sold[i] = a.cin[r0 + i];
-        }
+        } else if (isTile) {
+            const uint4 it = a.tiles[item - a.nlong];
+            r0 = it.x;
+            nrows = it.y - it.x;
+            const uint32_t e0 = it.z;
+            float val[kPerThread];
+            gather_tile<kHot>(a, e0, it.w, val, hot, tid);
+            // Row offsets (relative to the tile) and the rows' own degree
+            // and old contribution, loaded while the gathers are in flight.
+            for (uint32_t i = tid; i <= nrows; i += kBlock) L.roffs[i] = a.roff[r0 + i] - e0;
+            for (uint32_t i = tid; i < nrows; i += kBlock) {
+                L.sdeg[i] = a.deg[r0 + i];
+                L.sold[i] = a.cin[r0 + i];
+            }
 #pragma unroll
-        for (int k = 0; k < kPerThread; k++) vals[tid + kBlock * k] = val[k];
+            for (int k = 0; k < kPerThread; k++) L.vals[tid + kBlock * k] = val[k];
+        }
         __syncthreads();
-        // Rows with more than kBigRow edges are listed (in row order, by
-        // wave 0) and reduced by a whole wave each; the others by groups of
-        // tpr lanes, so one long row never serialises on a single lane.
-        if (wid == 0) {
+        // Rows with more than kBigRow edges are listed (in row order, by the
+        // group's first wave) and reduced by a whole wave each; the others
+        // by groups of tpr lanes, so one long row never serialises on a
+        // single lane.
+        if (isTile && wid == 0) {
             uint32_t count = 0;
             for (uint32_t b = 0; b < nrows; b += kWave) {
                 const uint32_t row = b + lane;
-                const bool big = row < nrows && roffs[row + 1] - roffs[row] > kBigRow;
+                const bool big = row < nrows && L.roffs[row + 1] - L.roffs[row] > kBigRow;
                 const uint64_t m = __ballot(big);
-                if (big) bigRows[count + __popcll(m & ((1ull << lane) - 1ull))] = row;
+                if (big) L.bigRows[count + __popcll(m & ((1ull << lane) - 1ull))] = row;
                 count += __popcll(m);
             }
-            if (lane == 0) nbig = count;
+            if (lane == 0) L.nbig = count;
         }
         __syncthreads();
-        // Lanes per row: the largest power of two <= 64 with nrows groups
-        // fitting the block; one lane per row (looping) beyond 256 rows.
-        uint32_t tpr = kWave;
-        while (tpr > 1 && tpr * nrows > kBlock) tpr >>= 1;
-        const uint32_t sub = tid % tpr;
-        for (uint32_t row = tid / tpr; row < nrows; row += kBlock / tpr) {
-            const uint32_t rs = roffs[row], re = roffs[row + 1];
-            if (tpr < kWave && re - rs > kBigRow) continue;  // whole group skips
-            Kahan k;
-            for (uint32_t j = rs + sub; j < re; j += tpr) k.add(vals[j]);
-            float ps = k.s, pc = k.c;
-            for (uint32_t o = tpr / 2; o > 0; o >>= 1) {
-                const float s2 = __shfl_xor(ps, static_cast<int>(o), kWave);
-                const float c2 = __shfl_xor(pc, static_cast<int>(o), kWave);
-                kahan_merge(ps, pc, s2, c2);
-            }
-            if (sub == 0) finalise(a, r0 + row, ps - pc, sdeg[row], sold[row], D, resid, dangl);
-        }
-        if (tpr < kWave) {
-            for (uint32_t b = wid; b < nbig; b += kBlock / kWave) {
-                const uint32_t row = bigRows[b];
-                const uint32_t rs = roffs[row], re = roffs[row + 1];
+        if (isTile) {
+            // Lanes per row: the largest power of two <= 64 with nrows groups
+            // fitting kBlock threads; one lane per row (looping) beyond.
+            uint32_t tpr = kWave;
+            while (tpr > 1 && tpr * nrows > kBlock) tpr >>= 1;
+            const uint32_t sub = tid % tpr;
+            for (uint32_t row = tid / tpr; row < nrows; row += kBlock / tpr) {
+                const uint32_t rs = L.roffs[row], re = L.roffs[row + 1];
+                if (tpr < kWave && re - rs > kBigRow) continue;  // whole group skips
                 Kahan k;
-                for (uint32_t j = rs + lane; j < re; j += kWave) k.add(vals[j]);
+                for (uint32_t j = rs + sub; j < re; j += tpr) k.add(L.vals[j]);
                 float ps = k.s, pc = k.c;
-                for (int o = kWave / 2; o > 0; o >>= 1) {
-                    const float s2 = __shfl_xor(ps, o, kWave), c2 = __shfl_xor(pc, o, kWave);
+                for (uint32_t o = tpr / 2; o > 0; o >>= 1) {
+                    const float s2 = __shfl_xor(ps, static_cast<int>(o), kWave);
+                    const float c2 = __shfl_xor(pc, static_cast<int>(o), kWave);
                     kahan_merge(ps, pc, s2, c2);
                 }
-                if (lane == 0) finalise(a, r0 + row, ps - pc, sdeg[row], sold[row], D, resid, dangl);
+                if (sub == 0) finalise(a, r0 + row, ps - pc, L.sdeg[row], L.sold[row], D, resid, dangl);
+            }
+            if (tpr < kWave) {
+                for (uint32_t b = wid; b < L.nbig; b += kBlock / kWave) {
+                    const uint32_t row = L.bigRows[b];
+                    const uint32_t rs = L.roffs[row], re = L.roffs[row + 1];
+                    Kahan k;
+                    for (uint32_t j = rs + lane; j < re; j += kWave) k.add(L.vals[j]);
+                    float ps = k.s, pc = k.c;
+                    for (int o = kWave / 2; o > 0; o >>= 1) {
+                        const float s2 = __shfl_xor(ps, o, kWave), c2 = __shfl_xor(pc, o, kWave);
+                        kahan_merge(ps, pc, s2, c2);
+                    }
+                    if (lane == 0) finalise(a, r0 + row, ps - pc, L.sdeg[row], L.sold[row], D, resid, dangl);
+                }
             }
         }
-    }
-    __syncthreads();  // the next item reuses the LDS staging
+        __syncthreads();  // the next step reuses the LDS staging
     }
 
     resid = wave_sum_f64(resid);
     dangl = wave_sum_f64(dangl);
+    const int gw = threadIdx.x / kWave;
     if (lane == 0) {
-        red[wid][0] = resid;
-        red[wid][1] = dangl;
+        red[gw][0] = resid;
+        red[gw][1] = dangl;
     }
     __syncthreads();
-    if (tid == 0) {
-        double r = 0.0, g = 0.0;
-        for (int w = 0; w < kBlock / kWave; w++) {
+    if (threadIdx.x == 0) {
+        double r = 0.0, q = 0.0;
+        for (int w = 0; w < kBlock * kSub / kWave; w++) {
             r += red[w][0];
-            g += red[w][1];
+            q += red[w][1];
         }
-        a.part[blockIdx.x] = make_double2(r, g);
+        a.part[blockIdx.x] = make_double2(r, q);
     }
 }
 
@@ -382,7 +403,8 @@ struct bw_pr_plan {
     uint32_t V = 0, E = 0;
     uint32_t ntiles = 0, nlong = 0, npieces = 0;
     uint32_t hot = 0;        // LDS hot-table size of the rows kernel (0, 8192, 16384)
-    uint32_t rows_grid = 0;  // rows-kernel workgroups (0: one per work item)
+    uint32_t rows_grid = 0;  // rows-kernel workgroups (0: one per step)
+    uint32_t sub = 1;        // row tiles per rows-kernel workgroup and step (1, 2 or 4)
     uint4 *pieces = nullptr;
     LongRow *lrows = nullptr;
     uint4 *tiles = nullptr;
@@ -489,13 +511,20 @@ extern "C" int bw_pr_plan_info(const bw_pr_plan *plan, uint32_t *ntiles, uint32_
     return BW_HIP_OK;
 }
 
-extern "C" int bw_pr_plan_tune(bw_pr_plan *plan, uint32_t hot_vertices, uint32_t rows_grid)
+extern "C" int bw_pr_plan_tune(bw_pr_plan *plan, uint32_t hot_vertices, uint32_t rows_grid,
+                               uint32_t tiles_per_workgroup)
 {
     if (!plan) return fail(BW_HIP_ERR_ARG, "plan is NULL");
-    if (hot_vertices != 0 && hot_vertices != 8192 && hot_vertices != 16384)
-        return fail(BW_HIP_ERR_ARG, "hot_vertices must be 0, 8192 or 16384");
-    plan->hot = hot_vertices;
+    const uint32_t h = hot_vertices, t = tiles_per_workgroup;
+    // The combinations whose LDS fits 160 KB per workgroup.
+    const bool ok = (t == 1 && (h == 0 || h == 8192 || h == 16384)) ||
+                    (t == 2 && (h == 0 || h == 8192 || h == 16384)) || (t == 4 && (h == 0 || h == 8192));
+    if (!ok)
+        return fail(BW_HIP_ERR_ARG, "unsupported launch shape: hot_vertices 0/8192/16384 with 1 or 2 tiles per "
+                                    "workgroup, 0/8192 with 4");
+    plan->hot = h;
     plan->rows_grid = rows_grid;
+    plan->sub = t;
     return BW_HIP_OK;
 }
 
@@ -543,11 +572,14 @@ extern "C" int bw_pr_fused_run(const bw_graph_dev *g, const bw_pr_plan *plan, in
         BW_HIP_CHECK(hipEventCreate(&ev1));
         BW_HIP_CHECK(hipEventRecord(ev0, s));
     }
-    const uint32_t items = plan->nlong + plan->ntiles;
-    const uint32_t rowsGrid = plan->rows_grid ? std::min(plan->rows_grid, items) : items;
-    auto rowsKernel = plan->hot == 16384 ? pr_rows_kernel<16384>
-                    : plan->hot == 8192  ? pr_rows_kernel<8192>
-                                         : pr_rows_kernel<0>;
+    const uint32_t steps = (plan->nlong + plan->ntiles + plan->sub - 1) / plan->sub;
+    const uint32_t rowsGrid = plan->rows_grid ? std::min(plan->rows_grid, steps) : steps;
+    auto rowsKernel = plan->sub == 4 ? (plan->hot == 8192 ? pr_rows_kernel<8192, 4> : pr_rows_kernel<0, 4>)
+                    : plan->sub == 2 ? (plan->hot == 16384 ? pr_rows_kernel<16384, 2>
+                                        : plan->hot == 8192 ? pr_rows_kernel<8192, 2> : pr_rows_kernel<0, 2>)
+                    : plan->hot == 16384 ? pr_rows_kernel<16384, 1>
+                    : plan->hot == 8192  ? pr_rows_kernel<8192, 1>
+                                         : pr_rows_kernel<0, 1>;
     auto piecesKernel = plan->hot == 16384 ? pr_pieces_kernel<16384>
                       : plan->hot == 8192  ? pr_pieces_kernel<8192>
                                            : pr_pieces_kernel<0>;
@@ -582,7 +614,7 @@ extern "C" int bw_pr_fused_run(const bw_graph_dev *g, const bw_pr_plan *plan, in
         hipLaunchKernelGGL(piecesKernel, dim3(piecesGrid), dim3(kBlock), 0, s, a);
         BW_LAUNCH_CHECK();
         if (rowsGrid) {
-            hipLaunchKernelGGL(rowsKernel, dim3(rowsGrid), dim3(kBlock), 0, s, a);
+            hipLaunchKernelGGL(rowsKernel, dim3(rowsGrid), dim3(kBlock * plan->sub), 0, s, a);
             BW_LAUNCH_CHECK();
         }
         nprev = rowsGrid;

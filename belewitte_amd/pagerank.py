@@ -122,33 +122,35 @@ class FusedPageRank:
                                           dg.desc.rev_offsets, 4, dg.desc.rev_cols))
         self._plan = p
         self.max_iterations = int(max_iterations)
-        self.tuning = (0, 0)
+        self.tuning = (0, 0, 1)
         V = max(dg.vertex_count, 1)
         self.ranks = DeviceArray(V, np.float32)
         self.scratch = DeviceArray(V, np.float32)
         self.acc = DeviceArray(int(lib.bw_pr_fused_acc_len(self.max_iterations)), np.float64)
 
-    def tune(self, hot: int = 0, rows_grid: int = 0) -> None:
-        """Launch shape of the row-tile kernel (never changes the ranks):
-        `hot` (0, 8192, 16384) most-gathered sources kept in LDS per
-        workgroup, `rows_grid` persistent workgroups (0 = one per item)."""
-        N.check_hip(N.hip_lib().bw_pr_plan_tune(self._plan, int(hot), int(rows_grid)))
-        self.tuning = (int(hot), int(rows_grid))
+    def tune(self, hot: int = 0, rows_grid: int = 0, tiles: int = 1) -> None:
+        """Launch shape of the row-tile kernel (DROP-mode ranks never
+        change): `hot` (0, 8192, 16384) most-gathered sources kept in LDS per
+        workgroup, `rows_grid` persistent workgroups (0 = one per step),
+        `tiles` (1, 2, 4) row tiles per workgroup and step."""
+        N.check_hip(N.hip_lib().bw_pr_plan_tune(self._plan, int(hot), int(rows_grid), int(tiles)))
+        self.tuning = (int(hot), int(rows_grid), int(tiles))
 
     def autotune(self, iterations: int = 10, candidates=None) -> dict:
         """Times each launch shape over `iterations` iterations and keeps
         the fastest; returns {shape: ms}."""
         from .device import device_props
         cus = device_props(0)["compute_units"]
-        cands = candidates or [(0, 0), (0, 5 * cus), (8192, 2 * cus), (16384, cus)]
+        cands = candidates or [(0, 0, 1), (0, 5 * cus, 1), (8192, 2 * cus, 1), (0, cus, 4), (8192, cus, 4),
+                               (8192, cus, 2), (16384, cus, 2)]
         table = {}
-        for hot, grid in cands:
-            self.tune(hot, grid)
+        for shape in cands:
+            self.tune(*shape)
             self.run(iterations, "uniform", download=False)
-            table[(hot, grid)] = min(self.run(iterations, "uniform", download=False)[2] for _ in range(3))
+            table[shape] = min(self.run(iterations, "uniform", download=False)[2] for _ in range(3))
         best = min(table, key=table.get)
         self.tune(*best)
-        return {f"hot{h}/grid{g}": round(v, 4) for (h, g), v in table.items()}
+        return {f"hot{h}/grid{g}/tiles{t}": round(v, 4) for (h, g, t), v in table.items()}
 
     def info(self) -> dict:
         a, b, c = C.c_uint32(), C.c_uint32(), C.c_uint32()

@@ -12,7 +12,12 @@ Prints one JSON line (rank 0).  Extra fields: the HBM roofline of the
 iteration kernel measured with HIP events on its stream, the CPU baseline
 (the oracle's OpenMP pull PageRank on the host cores, the same workload),
 the error of the GPU ranks against that baseline and against the FP64
-oracle, and the per-iteration rate of each of the reference's 10 variants.
+oracle, the per-iteration rate of each of the reference's 10 variants, and
+the BFS figures: the learned per-level switch against the best fixed
+variant and the per-level oracle on an undirected R-MAT scale 25 (GTEPS,
+with a CPU baseline), the batched searches (64 sources on R-MAT scale 20;
+256 small R-MAT graphs), and the reference's vertex-push on R-MAT scale 16
+(GPU and CPU).
 """
 from __future__ import annotations
 
@@ -101,6 +106,66 @@ def bfs_learned_switch(scale: int, sources: int, samples_out=None) -> dict:
     eng.close()
     dg.free()
     return res
+
+
+def bfs_reference_vertex_push(cpu: bool) -> dict:
+    """The reference's own BFS workload: vertex-push (all four frontier
+    reductions) on a directed R-MAT scale 16, edge factor 16 (seed 1), CSR
+    with uint64 offsets, from the max-out-degree vertex, with the reference
+    level loop (one frontier read-back per level).  Depths are checked
+    against the oracle; the CPU leg is the oracle's OpenMP vertex-push."""
+    from belewitte_amd import bfs, device, graph
+    from oracle import binding as O
+    g = graph.rmat(16, 16, seed=1)
+    root = g.max_out_degree_vertex()
+    dg = device.DeviceGraph(g, offset_bytes=8, reps=("csr",))
+    csr = O.CSR.from_graph(g)
+    want, _ = O.bfs_queue(csr, root)
+    reached = want != 2147483647
+    edges = int(np.diff(g.offsets()).astype(np.int64)[reached].sum())
+    out = {"graph": {"vertices": g.vertex_count, "edges": g.edge_count, "offset_bytes": 8}, "root": root}
+    for name in ("vertex-push", "vertex-push-bulk", "vertex-push-warpreduce", "vertex-push-blockreduce"):
+        bfs.run(dg, root, name)
+        best = np.inf
+        for _ in range(5):
+            t0 = time.perf_counter()
+            r = bfs.run(dg, root, name, time_levels=False)
+            best = min(best, time.perf_counter() - t0)
+        out[name] = {"ms": round(best * 1e3, 3), "gteps": round(edges / best / 1e9, 3),
+                     "bit_exact": bool(np.array_equal(r.levels, want))}
+    if cpu:
+        threads = min(len(os.sched_getaffinity(0)), 16)
+        for th in (threads, 1):
+            O.bfs_vertex_push_omp(csr, root, th)
+            best = np.inf
+            for _ in range(5):
+                t0 = time.perf_counter()
+                lv, _ = O.bfs_vertex_push_omp(csr, root, th)
+                best = min(best, time.perf_counter() - t0)
+            out[f"cpu_vertex_push_{th}_threads"] = {"ms": round(best * 1e3, 3),
+                                                    "gteps": round(edges / best / 1e9, 3),
+                                                    "bit_exact": bool(np.array_equal(lv, want))}
+    dg.free()
+    return out
+
+
+def bfs_cpu_baseline(scale: int) -> dict:
+    """The oracle's OpenMP vertex-push BFS (the reference kernel's level
+    loop restated) on the learned-switch graph from one source: a bounded
+    sample of the BFS workload on the host cores."""
+    from belewitte_amd import graph
+    from oracle import binding as O
+    g = graph.rmat(scale, 16, seed=5, undirected=True)
+    src = graph.random_sources(g, 1, seed=8)[0]
+    csr = O.CSR.from_graph(g)
+    threads = min(len(os.sched_getaffinity(0)), 16)
+    t0 = time.perf_counter()
+    lv, counts = O.bfs_vertex_push_omp(csr, src, threads)
+    dt = time.perf_counter() - t0
+    edges = int(np.diff(g.offsets()).astype(np.int64)[lv != 2147483647].sum()) // 2
+    return {"value": round(edges / dt / 1e9, 4), "unit": "GTEPS", "cores": threads, "kind": "port",
+            "sample": f"one source ({src}) of the 16, {len(counts)} levels, {dt:.2f} s, "
+                      f"undirected TEPS convention (reached out-degree sum / 2)"}
 
 
 def bfs_batched() -> dict:
@@ -226,6 +291,7 @@ def main() -> int:
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profile(),
                      "kernel": "pr_pieces_kernel + pr_rows_kernel (one iteration)",
                      "launch_shape": {"hot_vertices": fused.tuning[0], "rows_grid": fused.tuning[1],
+                                      "tiles_per_workgroup": fused.tuning[2],
                                       "autotune_ms_per_10_iterations": shapes},
                      "algorithmic_bytes_per_launch": bytes_per_launch,
                      "avg_launch_us": round(avg_launch_s * 1e6, 2)},
@@ -272,7 +338,11 @@ def main() -> int:
 
     if not args.no_bfs:
         out["bfs_learned_switch"] = bfs_learned_switch(args.bfs_scale, args.bfs_sources, args.bfs_samples_out)
+        if not args.no_cpu:
+            out["bfs_learned_switch"]["cpu_baseline"] = bfs_cpu_baseline(args.bfs_scale)
+            log(f"bfs cpu baseline {out['bfs_learned_switch']['cpu_baseline']}")
         out["bfs_batched"] = bfs_batched()
+        out["bfs_reference_vertex_push_rmat_s16"] = bfs_reference_vertex_push(not args.no_cpu)
 
     print(json.dumps(out), flush=True)
     return 0

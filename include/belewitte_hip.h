@@ -382,13 +382,15 @@ int bw_pr_plan_info(const bw_pr_plan *plan, uint32_t *row_tiles,
                     uint32_t *long_rows, uint32_t *long_pieces);
 /* Device out-degrees owned by the plan. */
 int bw_pr_plan_degrees(const bw_pr_plan *plan, const uint32_t **deg);
-/* Launch shape of the row-tile kernel (results are identical for every
- * setting): hot_vertices (0, 8192 or 16384) sources whose contributions
- * each workgroup keeps in LDS -- the most gathered ones when vertices are
- * numbered by decreasing out-degree; rows_grid workgroups looping over the
- * work items (0 = one workgroup per item). */
+/* Launch shape of the row-tile kernel (DROP-mode results are identical for
+ * every setting): hot_vertices (0, 8192 or 16384) sources whose
+ * contributions each workgroup keeps in LDS -- the most gathered ones when
+ * vertices are numbered by decreasing out-degree; rows_grid workgroups
+ * looping over the work items (0 = one workgroup per step);
+ * tiles_per_workgroup (1, 2 or 4) row tiles a workgroup processes side by
+ * side, sharing one hot table (4 only with hot_vertices 0 or 8192). */
 int bw_pr_plan_tune(bw_pr_plan *plan, uint32_t hot_vertices,
-                    uint32_t rows_grid);
+                    uint32_t rows_grid, uint32_t tiles_per_workgroup);
 
 /* Number of doubles of the `acc` workspace of bw_pr_fused_run. */
 uint64_t bw_pr_fused_acc_len(int32_t iterations);
