@@ -133,6 +133,19 @@ def bfs_reference_vertex_push(cpu: bool) -> dict:
             best = min(best, time.perf_counter() - t0)
         out[name] = {"ms": round(best * 1e3, 3), "gteps": round(edges / best / 1e9, 3),
                      "bit_exact": bool(np.array_equal(r.levels, want))}
+    # The same search by the device-driven engine (uint32 offsets, learned
+    # switch, level loop on the device), for contrast with the reference
+    # loop's per-level host round trip and thread-per-vertex expansion.
+    from belewitte_amd import bfs_native, switch
+    dg32 = device.DeviceGraph(g)
+    eng = bfs_native.BFSEngine(dg32)
+    eng.set_model(switch.Model.load(switch.MODEL_PATH).w)
+    eng.run(root, "learned", "td")
+    r = min((eng.run(root, "learned", "td") for _ in range(5)), key=lambda x: x.total_ms)
+    out["native_engine_learned"] = {"ms": round(r.total_ms, 3), "gteps": round(edges / (r.total_ms / 1e3) / 1e9, 3),
+                                    "bit_exact": bool(np.array_equal(r.levels, want)), "levels": r.num_levels}
+    eng.close()
+    dg32.free()
     if cpu:
         threads = min(len(os.sched_getaffinity(0)), 16)
         for th in (threads, 1):
