@@ -31,6 +31,7 @@
 #include "common.hpp"
 
 #include <algorithm>
+#include <thread>
 #include <vector>
 
 namespace bw {
@@ -84,6 +85,10 @@ struct IterArgs {
     const uint4 *tiles;      // (first row, end row, edge begin, edge end)
     float2 *partials;        // one Kahan partial per piece
     uint32_t npieces, nlong, ntiles;
+    const uint4 *windows;    // (first tile, tiles, edge begin, edge end), source-sorted mode
+    uint32_t nwin;
+    const uint32_t *scols;   // per window: column ids sorted by source
+    const uint16_t *sperm;   // per window: CSR position (within the window) of each sorted column
     const double2 *prev;     // per-workgroup (residual, dangling) of the previous launch
     uint32_t nprev;
     double *resid_out;       // reduced residual of the previous launch
@@ -223,15 +228,114 @@ __global__ __launch_bounds__(kBlock) void pr_pieces_kernel(IterArgs a)
     }
 }
 
-// LDS staging of one row tile.
-struct TileLds {
-    float vals[kTileNnz];
-    uint32_t roffs[kRowsMax + 1];
-    uint32_t sdeg[kRowsMax];
-    float sold[kRowsMax];
+// LDS staging of one row tile besides its gathered values: row offsets
+// relative to the tile's first edge (<= kTileNnz, so 16 bits) and the list
+// of big rows.  Kept small so that more workgroups fit a CU.
+struct TileMeta {
+    uint16_t roffs[kRowsMax + 1];
     uint32_t bigRows[kTileNnz / kBigRow];
     uint32_t nbig;
 };
+
+// Row offsets relative to the tile's first edge e0; `tid` indexes the
+// group's kBlock threads.
+__device__ __forceinline__ void stage_rows(const IterArgs &a, TileMeta &M, uint32_t r0, uint32_t nrows, uint32_t e0,
+                                           int tid)
+{
+    for (uint32_t i = tid; i <= nrows; i += kBlock) M.roffs[i] = static_cast<uint16_t>(a.roff[r0 + i] - e0);
+}
+
+// Rows with more than kBigRow edges, listed in row order by the group's
+// first wave.
+__device__ __forceinline__ void list_big_rows(TileMeta &M, uint32_t nrows, int lane)
+{
+    uint32_t count = 0;
+    for (uint32_t b = 0; b < nrows; b += kWave) {
+        const uint32_t row = b + lane;
+        const bool big = row < nrows && M.roffs[row + 1] - M.roffs[row] > kBigRow;
+        const uint64_t m = __ballot(big);
+        if (big) M.bigRows[count + __popcll(m & ((1ull << lane) - 1ull))] = row;
+        count += __popcll(m);
+    }
+    if (lane == 0) M.nbig = count;
+}
+
+// Finalises the tile's rows from the values in CSR order (`vals`): the big
+// rows by a whole wave each, the others by groups of tpr lanes, so one
+// long row never serialises on a single lane.  Each row's Kahan sum runs
+// over its edges in CSR order whatever the gather order was.
+__device__ __forceinline__ void reduce_rows(const IterArgs &a, const TileMeta &M, const float *vals, uint32_t r0,
+                                            uint32_t nrows, int tid, double D, double &resid, double &dangl)
+{
+    const int lane = tid & (kWave - 1), wid = tid / kWave;
+    // Lanes per row: the largest power of two <= 64 with nrows groups
+    // fitting kBlock threads; one lane per row (looping) beyond.
+    uint32_t tpr = kWave;
+    while (tpr > 1 && tpr * nrows > kBlock) tpr >>= 1;
+    const uint32_t sub = tid % tpr;
+    for (uint32_t row = tid / tpr; row < nrows; row += kBlock / tpr) {
+        const uint32_t rs = M.roffs[row], re = M.roffs[row + 1];
+        if (tpr < kWave && re - rs > kBigRow) continue;  // whole group skips
+        Kahan k;
+        for (uint32_t j = rs + sub; j < re; j += tpr) k.add(vals[j]);
+        float ps = k.s, pc = k.c;
+        for (uint32_t o = tpr / 2; o > 0; o >>= 1) {
+            const float s2 = __shfl_xor(ps, static_cast<int>(o), kWave);
+            const float c2 = __shfl_xor(pc, static_cast<int>(o), kWave);
+            kahan_merge(ps, pc, s2, c2);
+        }
+        if (sub == 0) finalise(a, r0 + row, ps - pc, a.deg[r0 + row], a.cin[r0 + row], D, resid, dangl);
+    }
+    if (tpr < kWave) {
+        for (uint32_t b = wid; b < M.nbig; b += kBlock / kWave) {
+            const uint32_t row = M.bigRows[b];
+            const uint32_t rs = M.roffs[row], re = M.roffs[row + 1];
+            Kahan k;
+            for (uint32_t j = rs + lane; j < re; j += kWave) k.add(vals[j]);
+            float ps = k.s, pc = k.c;
+            for (int o = kWave / 2; o > 0; o >>= 1) {
+                const float s2 = __shfl_xor(ps, o, kWave), c2 = __shfl_xor(pc, o, kWave);
+                kahan_merge(ps, pc, s2, c2);
+            }
+            if (lane == 0) finalise(a, r0 + row, ps - pc, a.deg[r0 + row], a.cin[r0 + row], D, resid, dangl);
+        }
+    }
+}
+
+// Long-row finaliser: combines the row's piece partials in piece order.
+__device__ __forceinline__ void finalise_long(const IterArgs &a, uint32_t item, double D, double &resid,
+                                              double &dangl)
+{
+    const LongRow lr = a.lrows[item];
+    float s = 0.0f, c = 0.0f;
+    for (uint32_t q = 0; q < lr.pieces; q++) {
+        const float2 p = a.partials[lr.base + q];
+        kahan_merge(s, c, p.x, p.y);
+    }
+    finalise(a, lr.row, s - c, a.deg[lr.row], a.cin[lr.row], D, resid, dangl);
+}
+
+// The workgroup's (residual, dangling) partial, fixed reduction order.
+template<int kThreads>
+__device__ __forceinline__ void write_partial(const IterArgs &a, double resid, double dangl, double (*red)[2])
+{
+    resid = wave_sum_f64(resid);
+    dangl = wave_sum_f64(dangl);
+    const int lane = threadIdx.x & (kWave - 1);
+    if (lane == 0) {
+        red[threadIdx.x / kWave][0] = resid;
+        red[threadIdx.x / kWave][1] = dangl;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double r = 0.0, q = 0.0;
+        for (int w = 0; w < kThreads / kWave; w++) {
+            r += red[w][0];
+            q += red[w][1];
+        }
+        a.part[blockIdx.x] = make_double2(r, q);
+    }
+}
 
 // Work items: nlong long-row finalisers, then the row tiles.  A workgroup
 // holds kSub groups of kBlock threads that take kSub consecutive items per
@@ -243,14 +347,15 @@ struct TileLds {
 template<uint32_t kHot, int kSub>
 __global__ __launch_bounds__(kBlock * kSub) void pr_rows_kernel(IterArgs a)
 {
-    __shared__ TileLds T[kSub];
+    __shared__ float vals[kSub][kTileNnz];
+    __shared__ TileMeta T[kSub];
     __shared__ double red[kBlock * kSub / kWave][2];
     __shared__ float hot[kHot > 0 ? kHot : 1];
 
     const int g = threadIdx.x / kBlock;   // group
     const int tid = threadIdx.x % kBlock;  // thread within the group
     const int lane = tid & (kWave - 1), wid = tid / kWave;
-    TileLds &L = T[g];
+    TileMeta &M = T[g];
     const double D = (a.mode == BW_PR_DANGLING_UNIFORM) ? *a.dangl : 0.0;
     double resid = 0.0, dangl = 0.0;
     if (kHot > 0) {
@@ -265,102 +370,87 @@ __global__ __launch_bounds__(kBlock * kSub) void pr_rows_kernel(IterArgs a)
         const bool isTile = item >= a.nlong && item < nitems;
         uint32_t r0 = 0, nrows = 0;
         if (item < a.nlong) {
-            if (tid == 0) {
-                const LongRow lr = a.lrows[item];
-                float s = 0.0f, c = 0.0f;
-                for (uint32_t q = 0; q < lr.pieces; q++) {
-                    const float2 p = a.partials[lr.base + q];
-                    kahan_merge(s, c, p.x, p.y);
-                }
-                finalise(a, lr.row, s - c, a.deg[lr.row], a.cin[lr.row], D, resid, dangl);
-            }
+            if (tid == 0) finalise_long(a, item, D, resid, dangl);
         } else if (isTile) {
             const uint4 it = a.tiles[item - a.nlong];
             r0 = it.x;
             nrows = it.y - it.x;
-            const uint32_t e0 = it.z;
             float val[kPerThread];
-            gather_tile<kHot>(a, e0, it.w, val, hot, tid);
-            // Row offsets (relative to the tile) and the rows' own degree
-            // and old contribution, loaded while the gathers are in flight.
-            for (uint32_t i = tid; i <= nrows; i += kBlock) L.roffs[i] = a.roff[r0 + i] - e0;
-            for (uint32_t i = tid; i < nrows; i += kBlock) {
-                L.sdeg[i] = a.deg[r0 + i];
-                L.sold[i] = a.cin[r0 + i];
-            }
+            gather_tile<kHot>(a, it.z, it.w, val, hot, tid);
+            stage_rows(a, M, r0, nrows, it.z, tid);  // while the gathers are in flight
 #pragma unroll
-            for (int k = 0; k < kPerThread; k++) L.vals[tid + kBlock * k] = val[k];
+            for (int k = 0; k < kPerThread; k++) vals[g][tid + kBlock * k] = val[k];
         }
         __syncthreads();
-        // Rows with more than kBigRow edges are listed (in row order, by the
-        // group's first wave) and reduced by a whole wave each; the others
-        // by groups of tpr lanes, so one long row never serialises on a
-        // single lane.
-        if (isTile && wid == 0) {
-            uint32_t count = 0;
-            for (uint32_t b = 0; b < nrows; b += kWave) {
-                const uint32_t row = b + lane;
-                const bool big = row < nrows && L.roffs[row + 1] - L.roffs[row] > kBigRow;
-                const uint64_t m = __ballot(big);
-                if (big) L.bigRows[count + __popcll(m & ((1ull << lane) - 1ull))] = row;
-                count += __popcll(m);
-            }
-            if (lane == 0) L.nbig = count;
-        }
+        if (isTile && wid == 0) list_big_rows(M, nrows, lane);
         __syncthreads();
-        if (isTile) {
-            // Lanes per row: the largest power of two <= 64 with nrows groups
-            // fitting kBlock threads; one lane per row (looping) beyond.
-            uint32_t tpr = kWave;
-            while (tpr > 1 && tpr * nrows > kBlock) tpr >>= 1;
-            const uint32_t sub = tid % tpr;
-            for (uint32_t row = tid / tpr; row < nrows; row += kBlock / tpr) {
-                const uint32_t rs = L.roffs[row], re = L.roffs[row + 1];
-                if (tpr < kWave && re - rs > kBigRow) continue;  // whole group skips
-                Kahan k;
-                for (uint32_t j = rs + sub; j < re; j += tpr) k.add(L.vals[j]);
-                float ps = k.s, pc = k.c;
-                for (uint32_t o = tpr / 2; o > 0; o >>= 1) {
-                    const float s2 = __shfl_xor(ps, static_cast<int>(o), kWave);
-                    const float c2 = __shfl_xor(pc, static_cast<int>(o), kWave);
-                    kahan_merge(ps, pc, s2, c2);
-                }
-                if (sub == 0) finalise(a, r0 + row, ps - pc, L.sdeg[row], L.sold[row], D, resid, dangl);
-            }
-            if (tpr < kWave) {
-                for (uint32_t b = wid; b < L.nbig; b += kBlock / kWave) {
-                    const uint32_t row = L.bigRows[b];
-                    const uint32_t rs = L.roffs[row], re = L.roffs[row + 1];
-                    Kahan k;
-                    for (uint32_t j = rs + lane; j < re; j += kWave) k.add(L.vals[j]);
-                    float ps = k.s, pc = k.c;
-                    for (int o = kWave / 2; o > 0; o >>= 1) {
-                        const float s2 = __shfl_xor(ps, o, kWave), c2 = __shfl_xor(pc, o, kWave);
-                        kahan_merge(ps, pc, s2, c2);
-                    }
-                    if (lane == 0) finalise(a, r0 + row, ps - pc, L.sdeg[row], L.sold[row], D, resid, dangl);
-                }
-            }
-        }
+        if (isTile) reduce_rows(a, M, vals[g], r0, nrows, tid, D, resid, dangl);
         __syncthreads();  // the next step reuses the LDS staging
     }
+    write_partial<kBlock * kSub>(a, resid, dangl, red);
+}
 
-    resid = wave_sum_f64(resid);
-    dangl = wave_sum_f64(dangl);
-    const int gw = threadIdx.x / kWave;
-    if (lane == 0) {
-        red[gw][0] = resid;
-        red[gw][1] = dangl;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double r = 0.0, q = 0.0;
-        for (int w = 0; w < kBlock * kSub / kWave; w++) {
-            r += red[w][0];
-            q += red[w][1];
+// Row tiles grouped into windows of up to kWinTiles consecutive tiles with
+// contiguous edges (<= kWinTiles * kTileNnz edges).  Within a window the
+// plan stores the column ids sorted by source (scols) and, for each, its
+// position in the window in CSR order (sperm): the gathers then run in
+// source order, so the lanes of one load hit far fewer distinct cache lines
+// (hub sources share lines), and each value is stored back at its CSR
+// position, so every row's sum is the same as in pr_rows_kernel.
+template<int kWinTiles>
+__global__ __launch_bounds__(kBlock * kWinTiles) void pr_windows_kernel(IterArgs a)
+{
+    constexpr uint32_t kWinNnz = kWinTiles * kTileNnz;
+    __shared__ float vals[kWinNnz];
+    __shared__ TileMeta T[kWinTiles];
+    __shared__ double red[kBlock * kWinTiles / kWave][2];
+
+    const int g = threadIdx.x / kBlock;
+    const int tid = threadIdx.x % kBlock;
+    const int lane = tid & (kWave - 1), wid = tid / kWave;
+    TileMeta &M = T[g];
+    const double D = (a.mode == BW_PR_DANGLING_UNIFORM) ? *a.dangl : 0.0;
+    double resid = 0.0, dangl = 0.0;
+
+    const uint32_t nitems = a.nlong + a.nwin;
+    for (uint32_t item = blockIdx.x; item < nitems; item += gridDim.x) {
+        bool isTile = false;
+        uint32_t r0 = 0, nrows = 0, voff = 0;
+        if (item < a.nlong) {
+            if (threadIdx.x == 0) finalise_long(a, item, D, resid, dangl);
+        } else {
+            const uint4 w = a.windows[item - a.nlong];  // first tile, tiles, edge begin, edge end
+            uint32_t col[kPerThread];
+            uint16_t pos[kPerThread];
+#pragma unroll
+            for (int k = 0; k < kPerThread; k++) {
+                const uint32_t j = w.z + threadIdx.x + kBlock * kWinTiles * k;
+                const bool in = j < w.w;
+                col[k] = in ? __builtin_nontemporal_load(&a.scols[j]) : 0xFFFFFFFFu;
+                pos[k] = in ? __builtin_nontemporal_load(&a.sperm[j]) : 0;
+            }
+            float val[kPerThread];
+#pragma unroll
+            for (int k = 0; k < kPerThread; k++) val[k] = col[k] != 0xFFFFFFFFu ? a.cin[col[k]] : 0.0f;
+            if (g < static_cast<int>(w.y)) {
+                const uint4 it = a.tiles[w.x + g];
+                isTile = true;
+                r0 = it.x;
+                nrows = it.y - it.x;
+                voff = it.z - w.z;
+                stage_rows(a, M, r0, nrows, it.z, tid);
+            }
+#pragma unroll
+            for (int k = 0; k < kPerThread; k++)
+                if (col[k] != 0xFFFFFFFFu) vals[pos[k]] = val[k];
         }
-        a.part[blockIdx.x] = make_double2(r, q);
+        __syncthreads();
+        if (isTile && wid == 0) list_big_rows(M, nrows, lane);
+        __syncthreads();
+        if (isTile) reduce_rows(a, M, vals + voff, r0, nrows, tid, D, resid, dangl);
+        __syncthreads();
     }
+    write_partial<kBlock * kWinTiles>(a, resid, dangl, red);
 }
 
 // Reduction of the last launch's partials (after the loop).
@@ -405,6 +495,14 @@ struct bw_pr_plan {
     uint32_t hot = 0;        // LDS hot-table size of the rows kernel (0, 8192, 16384)
     uint32_t rows_grid = 0;  // rows-kernel workgroups (0: one per step)
     uint32_t sub = 1;        // row tiles per rows-kernel workgroup and step (1, 2 or 4)
+    uint32_t sorted = 0;     // rows phase by source-sorted windows of this many tiles (0: off)
+    uint32_t sorted_width = 0;  // tiles per window of the built windows
+    const uint32_t *rcols = nullptr;  // the graph's reverse columns (for building the windows)
+    std::vector<uint4> tiles_host;
+    uint4 *windows = nullptr;
+    uint32_t nwin = 0;
+    uint32_t *scols = nullptr;
+    uint16_t *sperm = nullptr;
     uint4 *pieces = nullptr;
     LongRow *lrows = nullptr;
     uint4 *tiles = nullptr;
@@ -414,7 +512,8 @@ struct bw_pr_plan {
     ~bw_pr_plan()
     {
         for (void *p : {static_cast<void *>(pieces), static_cast<void *>(lrows), static_cast<void *>(tiles),
-                        static_cast<void *>(partials), static_cast<void *>(deg), static_cast<void *>(part)})
+                        static_cast<void *>(partials), static_cast<void *>(deg), static_cast<void *>(part),
+                        static_cast<void *>(windows), static_cast<void *>(scols), static_cast<void *>(sperm)})
             if (p) (void)hipFree(p);
     }
 };
@@ -462,6 +561,8 @@ extern "C" int bw_pr_plan_create(bw_pr_plan **out, uint64_t V, uint64_t E, const
         return fail(BW_HIP_ERR_ARG, "graph needs too many tiles");
 
     auto plan = new bw_pr_plan();
+    plan->rcols = rev_cols;
+    plan->tiles_host = tiles;
     plan->V = static_cast<uint32_t>(V);
     plan->E = static_cast<uint32_t>(E);
     plan->ntiles = static_cast<uint32_t>(tiles.size());
@@ -528,6 +629,87 @@ extern "C" int bw_pr_plan_tune(bw_pr_plan *plan, uint32_t hot_vertices, uint32_t
     return BW_HIP_OK;
 }
 
+extern "C" int bw_pr_plan_sort_windows(bw_pr_plan *plan, int32_t tiles_per_window)
+{
+    if (!plan) return fail(BW_HIP_ERR_ARG, "plan is NULL");
+    if (tiles_per_window == 0) {
+        plan->sorted = 0;
+        return BW_HIP_OK;
+    }
+    if (tiles_per_window != 1 && tiles_per_window != 2 && tiles_per_window != 4)
+        return fail(BW_HIP_ERR_ARG, "tiles_per_window must be 0, 1, 2 or 4");
+    const uint32_t width = static_cast<uint32_t>(tiles_per_window);
+    if (plan->windows && plan->sorted_width != width) {
+        (void)hipFree(plan->windows);
+        (void)hipFree(plan->scols);
+        (void)hipFree(plan->sperm);
+        plan->windows = nullptr;
+        plan->scols = nullptr;
+        plan->sperm = nullptr;
+    }
+    if (!plan->windows) {
+        // Windows: up to `width` consecutive tiles with contiguous edges.
+        const std::vector<uint4> &t = plan->tiles_host;
+        std::vector<uint4> win;
+        for (size_t i = 0; i < t.size();) {
+            size_t j = i + 1;
+            while (j < t.size() && j - i < width && t[j].z == t[j - 1].w && t[j].w - t[i].z <= width * kTileNnz)
+                j++;
+            win.push_back(make_uint4(static_cast<uint32_t>(i), static_cast<uint32_t>(j - i), t[i].z, t[j - 1].w));
+            i = j;
+        }
+        const uint64_t E = plan->E;
+        std::vector<uint32_t> cols(E), sc(std::max<uint64_t>(E, 1), 0);
+        std::vector<uint16_t> sp(std::max<uint64_t>(E, 1), 0);
+        if (E) BW_HIP_CHECK(hipMemcpy(cols.data(), plan->rcols, sizeof(uint32_t) * E, hipMemcpyDeviceToHost));
+        // Sort each window's edges by source (ties by position), in parallel.
+        const unsigned nth = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+        std::vector<std::thread> pool;
+        for (unsigned th = 0; th < nth; th++)
+            pool.emplace_back([&, th] {
+                std::vector<uint64_t> key;
+                for (size_t w = th; w < win.size(); w += nth) {
+                    const uint32_t e0 = win[w].z, n = win[w].w - win[w].z;
+                    key.resize(n);
+                    for (uint32_t k = 0; k < n; k++) key[k] = (static_cast<uint64_t>(cols[e0 + k]) << 16) | k;
+                    std::sort(key.begin(), key.end());
+                    for (uint32_t k = 0; k < n; k++) {
+                        sc[e0 + k] = static_cast<uint32_t>(key[k] >> 16);
+                        sp[e0 + k] = static_cast<uint16_t>(key[k] & 0xFFFFu);
+                    }
+                }
+            });
+        for (auto &th : pool) th.join();
+        uint4 *dw = nullptr;
+        uint32_t *dsc = nullptr;
+        uint16_t *dsp = nullptr;
+        auto release = [&] {
+            if (dw) (void)hipFree(dw);
+            if (dsc) (void)hipFree(dsc);
+            if (dsp) (void)hipFree(dsp);
+        };
+        if (hipMalloc(&dw, sizeof(uint4) * std::max<size_t>(win.size(), 1)) != hipSuccess ||
+            hipMalloc(&dsc, sizeof(uint32_t) * sc.size()) != hipSuccess ||
+            hipMalloc(&dsp, sizeof(uint16_t) * sp.size()) != hipSuccess) {
+            release();
+            return fail(BW_HIP_ERR_NOMEM, "device allocation of the sorted windows failed");
+        }
+        if ((!win.empty() && hipMemcpy(dw, win.data(), sizeof(uint4) * win.size(), hipMemcpyHostToDevice) != hipSuccess) ||
+            hipMemcpy(dsc, sc.data(), sizeof(uint32_t) * sc.size(), hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(dsp, sp.data(), sizeof(uint16_t) * sp.size(), hipMemcpyHostToDevice) != hipSuccess) {
+            release();
+            return fail(BW_HIP_ERR_RUNTIME, "upload of the sorted windows failed");
+        }
+        plan->windows = dw;
+        plan->scols = dsc;
+        plan->sperm = dsp;
+        plan->nwin = static_cast<uint32_t>(win.size());
+        plan->sorted_width = width;
+    }
+    plan->sorted = width;
+    return BW_HIP_OK;
+}
+
 extern "C" int bw_pr_plan_degrees(const bw_pr_plan *plan, const uint32_t **deg)
 {
     if (!plan || !deg) return fail(BW_HIP_ERR_ARG, "invalid argument");
@@ -572,7 +754,8 @@ extern "C" int bw_pr_fused_run(const bw_graph_dev *g, const bw_pr_plan *plan, in
         BW_HIP_CHECK(hipEventCreate(&ev1));
         BW_HIP_CHECK(hipEventRecord(ev0, s));
     }
-    const uint32_t steps = (plan->nlong + plan->ntiles + plan->sub - 1) / plan->sub;
+    const uint32_t steps = plan->sorted ? plan->nlong + plan->nwin
+                                        : (plan->nlong + plan->ntiles + plan->sub - 1) / plan->sub;
     const uint32_t rowsGrid = plan->rows_grid ? std::min(plan->rows_grid, steps) : steps;
     auto rowsKernel = plan->sub == 4 ? (plan->hot == 8192 ? pr_rows_kernel<8192, 4> : pr_rows_kernel<0, 4>)
                     : plan->sub == 2 ? (plan->hot == 16384 ? pr_rows_kernel<16384, 2>
@@ -602,6 +785,10 @@ extern "C" int bw_pr_fused_run(const bw_graph_dev *g, const bw_pr_plan *plan, in
         a.npieces = plan->npieces;
         a.nlong = plan->nlong;
         a.ntiles = plan->ntiles;
+        a.windows = plan->windows;
+        a.nwin = plan->nwin;
+        a.scols = plan->scols;
+        a.sperm = plan->sperm;
         a.prev = plan->part;
         a.nprev = nprev;
         a.resid_out = it > 1 ? resid + (it - 2) : spare;
@@ -614,7 +801,14 @@ extern "C" int bw_pr_fused_run(const bw_graph_dev *g, const bw_pr_plan *plan, in
         hipLaunchKernelGGL(piecesKernel, dim3(piecesGrid), dim3(kBlock), 0, s, a);
         BW_LAUNCH_CHECK();
         if (rowsGrid) {
-            hipLaunchKernelGGL(rowsKernel, dim3(rowsGrid), dim3(kBlock * plan->sub), 0, s, a);
+            if (plan->sorted == 4)
+                hipLaunchKernelGGL(pr_windows_kernel<4>, dim3(rowsGrid), dim3(kBlock * 4), 0, s, a);
+            else if (plan->sorted == 2)
+                hipLaunchKernelGGL(pr_windows_kernel<2>, dim3(rowsGrid), dim3(kBlock * 2), 0, s, a);
+            else if (plan->sorted == 1)
+                hipLaunchKernelGGL(pr_windows_kernel<1>, dim3(rowsGrid), dim3(kBlock), 0, s, a);
+            else
+                hipLaunchKernelGGL(rowsKernel, dim3(rowsGrid), dim3(kBlock * plan->sub), 0, s, a);
             BW_LAUNCH_CHECK();
         }
         nprev = rowsGrid;

@@ -4,7 +4,7 @@
         One streaming copy of the given size (bw_hip_copy_kernel): reads and
         writes exactly that many bytes, the reference point that scales the
         FETCH_SIZE / WRITE_SIZE counters to bytes.
-    python -m belewitte_amd.profiling pagerank --scale 22 --iterations 5 [--reorder] [--grid N --hot H --tiles T]
+    python -m belewitte_amd.profiling pagerank --scale 22 --iterations 5 [--reorder] [--grid N --hot H --tiles T] [--sorted W]
         A few fused PageRank iterations on the benchmark graph.
     python -m belewitte_amd.profiling bfs --scale 25 --sources 4 [--policy learned|td|bu|bu-wave]
         BFS engine searches on the undirected R-MAT of the learned-switch
@@ -35,7 +35,8 @@ def copy(mib: int, repeat: int) -> None:
     print(f"copy: {repeat} x {n * 4} bytes", flush=True)
 
 
-def pagerank(scale: int, iterations: int, reorder: bool, hot: int = 0, grid: int = 0, tiles: int = 1) -> None:
+def pagerank(scale: int, iterations: int, reorder: bool, hot: int = 0, grid: int = 0, tiles: int = 1,
+             sorted_windows: int = 0) -> None:
     from . import device, graph, pagerank as pr
     device.require_gpu()
     device.set_device(0)
@@ -44,7 +45,7 @@ def pagerank(scale: int, iterations: int, reorder: bool, hot: int = 0, grid: int
         g, _ = g.reorder_by_degree()
     dg = device.DeviceGraph(g, reps=("rev_csr",))
     f = pr.FusedPageRank(dg, max_iterations=iterations)
-    f.tune(hot, grid, tiles)
+    f.tune(hot, grid, tiles, sorted_windows)
     _, _, ms = f.run(iterations, "uniform", download=False)
     print(f"pagerank: V={g.vertex_count} E={g.edge_count} {iterations} iterations {ms:.3f} ms", flush=True)
 
@@ -80,6 +81,7 @@ def main(argv=None) -> int:
     p.add_argument("--hot", type=int, default=0)
     p.add_argument("--grid", type=int, default=0)
     p.add_argument("--tiles", type=int, default=1)
+    p.add_argument("--sorted", type=int, default=0, help="tiles per source-sorted window (0: off)")
     b = sub.add_parser("bfs")
     b.add_argument("--scale", type=int, default=25)
     b.add_argument("--sources", type=int, default=4)
@@ -88,7 +90,7 @@ def main(argv=None) -> int:
     if a.cmd == "copy":
         copy(a.mib, a.repeat)
     elif a.cmd == "pagerank":
-        pagerank(a.scale, a.iterations, a.reorder, a.hot, a.grid, a.tiles)
+        pagerank(a.scale, a.iterations, a.reorder, a.hot, a.grid, a.tiles, a.sorted)
     else:
         bfs(a.scale, a.sources, a.policy)
     return 0

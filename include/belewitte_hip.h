@@ -391,6 +391,15 @@ int bw_pr_plan_degrees(const bw_pr_plan *plan, const uint32_t **deg);
  * side, sharing one hot table (4 only with hot_vertices 0 or 8192). */
 int bw_pr_plan_tune(bw_pr_plan *plan, uint32_t hot_vertices,
                     uint32_t rows_grid, uint32_t tiles_per_workgroup);
+/* tiles_per_window 1, 2 or 4: the row tiles are processed in windows of
+ * up to that many consecutive tiles whose gathers run in source order
+ * (built on the first call for a width: the window edges sorted by source
+ * id with their CSR positions, 6 extra bytes per edge of device memory;
+ * reads the graph's reverse columns given to bw_pr_plan_create, which must
+ * still be live).  Every row is still summed in CSR order, so the ranks do
+ * not change; rows_grid applies to the window kernel too.  0 returns to
+ * the tile kernel. */
+int bw_pr_plan_sort_windows(bw_pr_plan *plan, int32_t tiles_per_window);
 
 /* Number of doubles of the `acc` workspace of bw_pr_fused_run. */
 uint64_t bw_pr_fused_acc_len(int32_t iterations);

@@ -134,10 +134,12 @@ def test_fused_degree_ordered_matches_oracle(gpu):
     fp.close()
 
 
-@pytest.mark.parametrize("hot,grid,tiles", [(0, 300, 1), (8192, 512, 1), (16384, 256, 1), (8192, 0, 1),
-                                            (16384, 7, 1), (0, 256, 4), (8192, 256, 4), (8192, 0, 4),
-                                            (8192, 64, 2), (16384, 3, 2), (0, 0, 2)])
-def test_fused_launch_shapes(gpu, hot, grid, tiles):
+@pytest.mark.parametrize("hot,grid,tiles,srt", [(0, 300, 1, 0), (8192, 512, 1, 0), (16384, 256, 1, 0),
+                                                (8192, 0, 1, 0), (16384, 7, 1, 0), (0, 256, 4, 0),
+                                                (8192, 256, 4, 0), (8192, 0, 4, 0), (8192, 64, 2, 0),
+                                                (16384, 3, 2, 0), (0, 0, 2, 0), (0, 0, 1, 4),
+                                                (0, 256, 1, 4), (0, 5, 1, 2), (0, 0, 1, 1), (0, 700, 1, 2)])
+def test_fused_launch_shapes(gpu, hot, grid, tiles, srt):
     """Every launch shape of the row-tile kernel (LDS hot table, persistent
     grid) yields the same ranks: bit for bit in DROP mode; in UNIFORM mode
     the FP64 dangling mass is summed over per-workgroup partials, so a
@@ -146,10 +148,10 @@ def test_fused_launch_shapes(gpu, hot, grid, tiles):
         fp = pagerank.FusedPageRank.for_graph(g, max_iterations=20, reorder=True)
         base, rb, _ = fp.run(20, "drop")
         ubase, _, _ = fp.run(20, "uniform")
-        fp.tune(hot, grid, tiles)
+        fp.tune(hot, grid, tiles, srt)
         got, rg, _ = fp.run(20, "drop")
-        assert np.array_equal(base.view(np.uint32), got.view(np.uint32)), (g.name, hot, grid, tiles)
+        assert np.array_equal(base.view(np.uint32), got.view(np.uint32)), (g.name, hot, grid, tiles, srt)
         assert np.allclose(rb, rg, rtol=1e-12, atol=0)
         ugot, _, _ = fp.run(20, "uniform")
-        assert np.max(np.abs(ugot - ubase)) <= 4e-7 * ubase.max(), (g.name, hot, grid, tiles)
+        assert np.max(np.abs(ugot - ubase)) <= 4e-7 * ubase.max(), (g.name, hot, grid, tiles, srt)
         fp.close()
