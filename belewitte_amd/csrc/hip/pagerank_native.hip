@@ -397,7 +397,7 @@ __global__ __launch_bounds__(kBlock * kSub) void pr_rows_kernel(IterArgs a)
 // source order, so the lanes of one load hit far fewer distinct cache lines
 // (hub sources share lines), and each value is stored back at its CSR
 // position, so every row's sum is the same as in pr_rows_kernel.
-template<int kWinTiles>
+template<int kWinTiles, int kWinBatch>
 __global__ __launch_bounds__(kBlock * kWinTiles) void pr_windows_kernel(IterArgs a)
 {
     constexpr uint32_t kWinNnz = kWinTiles * kTileNnz;
@@ -420,18 +420,6 @@ __global__ __launch_bounds__(kBlock * kWinTiles) void pr_windows_kernel(IterArgs
             if (threadIdx.x == 0) finalise_long(a, item, D, resid, dangl);
         } else {
             const uint4 w = a.windows[item - a.nlong];  // first tile, tiles, edge begin, edge end
-            uint32_t col[kPerThread];
-            uint16_t pos[kPerThread];
-#pragma unroll
-            for (int k = 0; k < kPerThread; k++) {
-                const uint32_t j = w.z + threadIdx.x + kBlock * kWinTiles * k;
-                const bool in = j < w.w;
-                col[k] = in ? __builtin_nontemporal_load(&a.scols[j]) : 0xFFFFFFFFu;
-                pos[k] = in ? __builtin_nontemporal_load(&a.sperm[j]) : 0;
-            }
-            float val[kPerThread];
-#pragma unroll
-            for (int k = 0; k < kPerThread; k++) val[k] = col[k] != 0xFFFFFFFFu ? a.cin[col[k]] : 0.0f;
             if (g < static_cast<int>(w.y)) {
                 const uint4 it = a.tiles[w.x + g];
                 isTile = true;
@@ -440,9 +428,26 @@ __global__ __launch_bounds__(kBlock * kWinTiles) void pr_windows_kernel(IterArgs
                 voff = it.z - w.z;
                 stage_rows(a, M, r0, nrows, it.z, tid);
             }
+            // kWinBatch gathers in flight per thread and round: a smaller
+            // batch holds fewer registers, so more waves fit a CU.
 #pragma unroll
-            for (int k = 0; k < kPerThread; k++)
-                if (col[k] != 0xFFFFFFFFu) vals[pos[k]] = val[k];
+            for (int r = 0; r < kPerThread / kWinBatch; r++) {
+                uint32_t col[kWinBatch];
+                uint16_t pos[kWinBatch];
+#pragma unroll
+                for (int k = 0; k < kWinBatch; k++) {
+                    const uint32_t j = w.z + threadIdx.x + kBlock * kWinTiles * (r * kWinBatch + k);
+                    const bool in = j < w.w;
+                    col[k] = in ? __builtin_nontemporal_load(&a.scols[j]) : 0xFFFFFFFFu;
+                    pos[k] = in ? __builtin_nontemporal_load(&a.sperm[j]) : 0;
+                }
+                float val[kWinBatch];
+#pragma unroll
+                for (int k = 0; k < kWinBatch; k++) val[k] = col[k] != 0xFFFFFFFFu ? a.cin[col[k]] : 0.0f;
+#pragma unroll
+                for (int k = 0; k < kWinBatch; k++)
+                    if (col[k] != 0xFFFFFFFFu) vals[pos[k]] = val[k];
+            }
         }
         __syncthreads();
         if (isTile && wid == 0) list_big_rows(M, nrows, lane);
@@ -497,6 +502,7 @@ struct bw_pr_plan {
     uint32_t sub = 1;        // row tiles per rows-kernel workgroup and step (1, 2 or 4)
     uint32_t sorted = 0;     // rows phase by source-sorted windows of this many tiles (0: off)
     uint32_t sorted_width = 0;  // tiles per window of the built windows
+    uint32_t win_batch = 16;    // gathers in flight per thread in the window kernel (8 or 16)
     const uint32_t *rcols = nullptr;  // the graph's reverse columns (for building the windows)
     std::vector<uint4> tiles_host;
     uint4 *windows = nullptr;
@@ -629,9 +635,12 @@ extern "C" int bw_pr_plan_tune(bw_pr_plan *plan, uint32_t hot_vertices, uint32_t
     return BW_HIP_OK;
 }
 
-extern "C" int bw_pr_plan_sort_windows(bw_pr_plan *plan, int32_t tiles_per_window)
+extern "C" int bw_pr_plan_sort_windows(bw_pr_plan *plan, int32_t tiles_per_window, int32_t gathers_per_round)
 {
     if (!plan) return fail(BW_HIP_ERR_ARG, "plan is NULL");
+    if (gathers_per_round != 8 && gathers_per_round != 16)
+        return fail(BW_HIP_ERR_ARG, "gathers_per_round must be 8 or 16");
+    plan->win_batch = static_cast<uint32_t>(gathers_per_round);
     if (tiles_per_window == 0) {
         plan->sorted = 0;
         return BW_HIP_OK;
@@ -801,13 +810,14 @@ extern "C" int bw_pr_fused_run(const bw_graph_dev *g, const bw_pr_plan *plan, in
         hipLaunchKernelGGL(piecesKernel, dim3(piecesGrid), dim3(kBlock), 0, s, a);
         BW_LAUNCH_CHECK();
         if (rowsGrid) {
-            if (plan->sorted == 4)
-                hipLaunchKernelGGL(pr_windows_kernel<4>, dim3(rowsGrid), dim3(kBlock * 4), 0, s, a);
-            else if (plan->sorted == 2)
-                hipLaunchKernelGGL(pr_windows_kernel<2>, dim3(rowsGrid), dim3(kBlock * 2), 0, s, a);
-            else if (plan->sorted == 1)
-                hipLaunchKernelGGL(pr_windows_kernel<1>, dim3(rowsGrid), dim3(kBlock), 0, s, a);
-            else
+            if (plan->sorted) {
+                using WinKernel = void (*)(IterArgs);
+                const bool b8 = plan->win_batch == 8;
+                WinKernel k = plan->sorted == 4 ? (b8 ? pr_windows_kernel<4, 8> : pr_windows_kernel<4, 16>)
+                            : plan->sorted == 2 ? (b8 ? pr_windows_kernel<2, 8> : pr_windows_kernel<2, 16>)
+                                                : (b8 ? pr_windows_kernel<1, 8> : pr_windows_kernel<1, 16>);
+                hipLaunchKernelGGL(k, dim3(rowsGrid), dim3(kBlock * plan->sorted), 0, s, a);
+            } else
                 hipLaunchKernelGGL(rowsKernel, dim3(rowsGrid), dim3(kBlock * plan->sub), 0, s, a);
             BW_LAUNCH_CHECK();
         }

@@ -122,31 +122,34 @@ class FusedPageRank:
                                           dg.desc.rev_offsets, 4, dg.desc.rev_cols))
         self._plan = p
         self.max_iterations = int(max_iterations)
-        self.tuning = (0, 0, 1, 0)
+        self.tuning = (0, 0, 1, 0, 16)
         V = max(dg.vertex_count, 1)
         self.ranks = DeviceArray(V, np.float32)
         self.scratch = DeviceArray(V, np.float32)
         self.acc = DeviceArray(int(lib.bw_pr_fused_acc_len(self.max_iterations)), np.float64)
 
-    def tune(self, hot: int = 0, rows_grid: int = 0, tiles: int = 1, sorted_windows: int = 0) -> None:
+    def tune(self, hot: int = 0, rows_grid: int = 0, tiles: int = 1, sorted_windows: int = 0,
+             batch: int = 16) -> None:
         """Launch shape of the row phase (DROP-mode ranks never change):
         `hot` (0, 8192, 16384) most-gathered sources kept in LDS per
         workgroup, `rows_grid` persistent workgroups (0 = one per step),
         `tiles` (1, 2, 4) row tiles per workgroup and step, or
         `sorted_windows` (1, 2, 4): windows of that many tiles gathered in
-        source order (0: off)."""
+        source order (0: off), `batch` (8, 16) gathers in flight per thread
+        in the window kernel."""
         lib = N.hip_lib()
         N.check_hip(lib.bw_pr_plan_tune(self._plan, int(hot), int(rows_grid), int(tiles)))
-        N.check_hip(lib.bw_pr_plan_sort_windows(self._plan, int(sorted_windows)))
-        self.tuning = (int(hot), int(rows_grid), int(tiles), int(sorted_windows))
+        N.check_hip(lib.bw_pr_plan_sort_windows(self._plan, int(sorted_windows), int(batch)))
+        self.tuning = (int(hot), int(rows_grid), int(tiles), int(sorted_windows), int(batch))
 
     def autotune(self, iterations: int = 10, candidates=None) -> dict:
         """Times each launch shape over `iterations` iterations and keeps
         the fastest; returns {shape: ms}."""
         from .device import device_props
         cus = device_props(0)["compute_units"]
-        cands = candidates or [(0, 8 * cus, 1, 0), (0, 2 * cus, 1, 4), (0, 4 * cus, 1, 2), (0, 3 * cus, 1, 2),
-                               (0, 6 * cus, 1, 1), (0, 8 * cus, 1, 1), (0, 0, 1, 1)]
+        cands = candidates or [(0, 8 * cus, 1, 0, 16), (0, 2 * cus, 1, 4, 8), (0, 2 * cus, 1, 4, 16),
+                               (0, 3 * cus, 1, 2, 16), (0, 4 * cus, 1, 2, 8), (0, 4 * cus, 1, 2, 16),
+                               (0, 8 * cus, 1, 1, 8)]
         table = {}
         for shape in cands:
             self.tune(*shape)
@@ -154,8 +157,8 @@ class FusedPageRank:
             table[shape] = min(self.run(iterations, "uniform", download=False)[2] for _ in range(3))
         best = min(table, key=table.get)
         self.tune(*best)
-        return {f"hot{h}/grid{g}/tiles{t}" + (f"/sorted{w}" if w else ""): round(v, 4)
-                for (h, g, t, w), v in table.items()}
+        return {f"hot{h}/grid{g}/tiles{t}" + (f"/sorted{w}/batch{b}" if w else ""): round(v, 4)
+                for (h, g, t, w, b), v in table.items()}
 
     def info(self) -> dict:
         a, b, c = C.c_uint32(), C.c_uint32(), C.c_uint32()

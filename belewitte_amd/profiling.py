@@ -4,7 +4,7 @@
         One streaming copy of the given size (bw_hip_copy_kernel): reads and
         writes exactly that many bytes, the reference point that scales the
         FETCH_SIZE / WRITE_SIZE counters to bytes.
-    python -m belewitte_amd.profiling pagerank --scale 22 --iterations 5 [--reorder] [--grid N --hot H --tiles T] [--sorted W]
+    python -m belewitte_amd.profiling pagerank --scale 22 --iterations 5 [--reorder] [--grid N --hot H --tiles T] [--sorted W --batch B]
         A few fused PageRank iterations on the benchmark graph.
     python -m belewitte_amd.profiling bfs --scale 25 --sources 4 [--policy learned|td|bu|bu-wave]
         BFS engine searches on the undirected R-MAT of the learned-switch
@@ -36,7 +36,7 @@ def copy(mib: int, repeat: int) -> None:
 
 
 def pagerank(scale: int, iterations: int, reorder: bool, hot: int = 0, grid: int = 0, tiles: int = 1,
-             sorted_windows: int = 0) -> None:
+             sorted_windows: int = 0, batch: int = 16) -> None:
     from . import device, graph, pagerank as pr
     device.require_gpu()
     device.set_device(0)
@@ -45,7 +45,7 @@ def pagerank(scale: int, iterations: int, reorder: bool, hot: int = 0, grid: int
         g, _ = g.reorder_by_degree()
     dg = device.DeviceGraph(g, reps=("rev_csr",))
     f = pr.FusedPageRank(dg, max_iterations=iterations)
-    f.tune(hot, grid, tiles, sorted_windows)
+    f.tune(hot, grid, tiles, sorted_windows, batch)
     _, _, ms = f.run(iterations, "uniform", download=False)
     print(f"pagerank: V={g.vertex_count} E={g.edge_count} {iterations} iterations {ms:.3f} ms", flush=True)
 
@@ -82,6 +82,7 @@ def main(argv=None) -> int:
     p.add_argument("--grid", type=int, default=0)
     p.add_argument("--tiles", type=int, default=1)
     p.add_argument("--sorted", type=int, default=0, help="tiles per source-sorted window (0: off)")
+    p.add_argument("--batch", type=int, default=16, help="gathers in flight per thread (window kernel)")
     b = sub.add_parser("bfs")
     b.add_argument("--scale", type=int, default=25)
     b.add_argument("--sources", type=int, default=4)
@@ -90,7 +91,7 @@ def main(argv=None) -> int:
     if a.cmd == "copy":
         copy(a.mib, a.repeat)
     elif a.cmd == "pagerank":
-        pagerank(a.scale, a.iterations, a.reorder, a.hot, a.grid, a.tiles, a.sorted)
+        pagerank(a.scale, a.iterations, a.reorder, a.hot, a.grid, a.tiles, a.sorted, a.batch)
     else:
         bfs(a.scale, a.sources, a.policy)
     return 0

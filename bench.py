@@ -306,6 +306,7 @@ def main() -> int:
                      "launch_shape": {"hot_vertices": fused.tuning[0], "rows_grid": fused.tuning[1],
                                       "tiles_per_workgroup": fused.tuning[2],
                                       "source_sorted_windows": fused.tuning[3],
+                                      "window_gathers_per_round": fused.tuning[4],
                                       "autotune_ms_per_10_iterations": shapes},
                      "algorithmic_bytes_per_launch": bytes_per_launch,
                      "avg_launch_us": round(avg_launch_s * 1e6, 2)},

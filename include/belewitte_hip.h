@@ -398,8 +398,10 @@ int bw_pr_plan_tune(bw_pr_plan *plan, uint32_t hot_vertices,
  * reads the graph's reverse columns given to bw_pr_plan_create, which must
  * still be live).  Every row is still summed in CSR order, so the ranks do
  * not change; rows_grid applies to the window kernel too.  0 returns to
- * the tile kernel. */
-int bw_pr_plan_sort_windows(bw_pr_plan *plan, int32_t tiles_per_window);
+ * the tile kernel.  gathers_per_round (8 or 16): gathers each thread keeps
+ * in flight at once (8 needs fewer registers, so more waves fit a CU). */
+int bw_pr_plan_sort_windows(bw_pr_plan *plan, int32_t tiles_per_window,
+                            int32_t gathers_per_round);
 
 /* Number of doubles of the `acc` workspace of bw_pr_fused_run. */
 uint64_t bw_pr_fused_acc_len(int32_t iterations);
