@@ -32,7 +32,8 @@ ARCH = os.environ.get("BW_OFFLOAD_ARCH", "gfx950")
 
 HIP_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC",
-    # Packed FP32 instructions stay off on this target.
+    # Packed FP32 instructions stay off: some of their forms have given wrong
+    # results on this target, with no error.
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall", "-Wno-unused-command-line-argument",
     f"-I{INC}",

@@ -97,6 +97,33 @@ __global__ void copy_u32_tail_kernel(uint32_t *dst, const uint32_t *src, uint64_
     if (i < n) dst[i] = src[i];
 }
 
+// Gather-rate probe: thread t sums table[idx[r * threads + t]] over its
+// rounds, kGatherBatch loads in flight at a time, so consecutive lanes of a
+// wave read consecutive index entries (the access shape of a row tile's
+// gathers).  Indices at or beyond `len` read nothing.
+constexpr int kGatherBatch = 16;
+
+__global__ __launch_bounds__(256) void gather_probe_kernel(const uint32_t *__restrict__ idx,
+                                                           const float *__restrict__ table, uint32_t len,
+                                                           float *__restrict__ out, uint32_t rounds)
+{
+    const uint64_t threads = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+    const uint64_t t = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    float acc = 0.0f;
+    for (uint32_t r = 0; r < rounds; r++) {
+        uint32_t c[kGatherBatch];
+#pragma unroll
+        for (int k = 0; k < kGatherBatch; k++)
+            c[k] = __builtin_nontemporal_load(&idx[(static_cast<uint64_t>(r) * kGatherBatch + k) * threads + t]);
+        float v[kGatherBatch];
+#pragma unroll
+        for (int k = 0; k < kGatherBatch; k++) v[k] = c[k] < len ? table[c[k]] : 0.0f;
+#pragma unroll
+        for (int k = 0; k < kGatherBatch; k++) acc += v[k];
+    }
+    out[t] = acc;
+}
+
 }  // namespace bw
 
 using namespace bw;
@@ -244,6 +271,34 @@ extern "C" int bw_hip_copy_kernel(uint32_t *dst, const uint32_t *src, uint64_t n
     if (n4 * 4 < n) {
         hipLaunchKernelGGL(copy_u32_tail_kernel, dim3(1), dim3(64), 0, s, dst, src, n4 * 4, n);
         BW_LAUNCH_CHECK();
+    }
+    return BW_HIP_OK;
+}
+
+extern "C" int bw_hip_gather_probe(const uint32_t *idx, uint64_t n, const float *table, uint32_t table_len,
+                                   float *out, uint32_t threads, float *kernel_ms, void *stream)
+{
+    if (!idx || !table || !out || table_len == 0) return fail(BW_HIP_ERR_ARG, "gather_probe: invalid argument");
+    if (threads == 0 || threads % 256 != 0) return fail(BW_HIP_ERR_ARG, "gather_probe: threads must be a multiple of 256");
+    const uint64_t per_round = static_cast<uint64_t>(threads) * kGatherBatch;
+    if (n == 0 || n % per_round != 0 || n / per_round > 0xFFFFFFFFull)
+        return fail(BW_HIP_ERR_ARG, "gather_probe: n must be a positive multiple of 16 * threads");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (kernel_ms) {
+        BW_HIP_CHECK(hipEventCreate(&e0));
+        BW_HIP_CHECK(hipEventCreate(&e1));
+        BW_HIP_CHECK(hipEventRecord(e0, s));
+    }
+    hipLaunchKernelGGL(gather_probe_kernel, dim3(threads / 256), dim3(256), 0, s, idx, table, table_len, out,
+                       static_cast<uint32_t>(n / per_round));
+    BW_LAUNCH_CHECK();
+    if (kernel_ms) {
+        BW_HIP_CHECK(hipEventRecord(e1, s));
+        BW_HIP_CHECK(hipEventSynchronize(e1));
+        BW_HIP_CHECK(hipEventElapsedTime(kernel_ms, e0, e1));
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
     }
     return BW_HIP_OK;
 }

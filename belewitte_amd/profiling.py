@@ -4,6 +4,10 @@
         One streaming copy of the given size (bw_hip_copy_kernel): reads and
         writes exactly that many bytes, the reference point that scales the
         FETCH_SIZE / WRITE_SIZE counters to bytes.
+    python -m belewitte_amd.profiling gather --table-mib 16 --share 1,2,4,8,16,32,64
+        The gather-rate probe (bw_hip_gather_probe): 64 Mi gathers from a
+        float table where `share` consecutive lanes read one 128-byte line;
+        prints lanes/s and distinct lines/s per sharing factor.
     python -m belewitte_amd.profiling pagerank --scale 22 --iterations 5 [--reorder] [--grid N --hot H --tiles T] [--sorted W --batch B]
         A few fused PageRank iterations on the benchmark graph.
     python -m belewitte_amd.profiling bfs --scale 25 --sources 4 [--policy learned|td|bu|bu-wave]
@@ -33,6 +37,58 @@ def copy(mib: int, repeat: int) -> None:
         N.check_hip(N.hip_lib().bw_hip_copy_kernel(dst.ptr, src.ptr, n, None))
     synchronize()
     print(f"copy: {repeat} x {n * 4} bytes", flush=True)
+
+
+LINE_FLOATS = 32  # floats per 128-byte cache line
+
+
+def gather_indices(n: int, table_len: int, share: int, seed: int = 0, stride: int = 1) -> np.ndarray:
+    """Index stream of the gather probe: groups of `share` consecutive
+    entries read one random line of a `table_len`-float table; entry i
+    reads word (i * stride) % 32 of it (stride 1: consecutive words,
+    2: every other word, 0: all the same word)."""
+    if share < 1 or share > 64 or 64 % share or table_len < LINE_FLOATS or stride < 0:
+        raise ValueError("share must divide 64 and the table hold one line")
+    rng = np.random.default_rng(seed)
+    lines = rng.integers(0, table_len // LINE_FLOATS, size=n // share, dtype=np.int64)
+    words = (np.arange(n, dtype=np.int64) * stride) % LINE_FLOATS
+    return (np.repeat(lines, share) * LINE_FLOATS + words).astype(np.uint32)
+
+
+def gather(table_mib: int, shares, mi: int = 64, threads: int = 256 * 8192, repeat: int = 5,
+           stride: int = 1) -> dict:
+    import ctypes as C
+
+    from . import _native as N
+    from .device import DeviceArray, require_gpu, set_device
+    require_gpu()
+    set_device(0)
+    n = mi << 20
+    threads = min(threads, n // 16)  # at least one round of 16 gathers per thread
+    if threads % 256 or n % (16 * threads):
+        raise ValueError("--mi must be a multiple of 32 or divide it")
+    tlen = table_mib * (1 << 20) // 4
+    table = DeviceArray(tlen, np.float32)
+    table.upload((np.arange(tlen) % 7).astype(np.float32))
+    idx = DeviceArray(n, np.uint32)
+    out = DeviceArray(threads, np.float32)
+    res = {}
+    for share in shares:
+        idx.upload(gather_indices(n, tlen, share, stride=stride))
+        best = float("inf")
+        for _ in range(repeat):
+            ms = C.c_float()
+            N.check_hip(N.hip_lib().bw_hip_gather_probe(idx.ptr, n, table.ptr, tlen, out.ptr, threads,
+                                                        C.byref(ms), None))
+            best = min(best, ms.value)
+        lanes = n / best / 1e6
+        res[share] = {"ms": round(best, 4), "G_lanes_per_s": round(lanes, 1),
+                      "G_lines_per_s": round(lanes / share, 1)}
+        print(f"gather: table {table_mib} MiB, {share:2d} lanes per line, word stride {stride}: {best:.3f} ms, "
+              f"{lanes:.1f} G lanes/s, {lanes / share:.1f} G lines/s", flush=True)
+    for a in (table, idx, out):
+        a.free()
+    return res
 
 
 def pagerank(scale: int, iterations: int, reorder: bool, hot: int = 0, grid: int = 0, tiles: int = 1,
@@ -83,6 +139,11 @@ def main(argv=None) -> int:
     p.add_argument("--tiles", type=int, default=1)
     p.add_argument("--sorted", type=int, default=0, help="tiles per source-sorted window (0: off)")
     p.add_argument("--batch", type=int, default=16, help="gathers in flight per thread (window kernel)")
+    q = sub.add_parser("gather")
+    q.add_argument("--table-mib", type=int, default=16)
+    q.add_argument("--share", default="1,2,4,8,16,32,64", help="lanes per line, comma separated")
+    q.add_argument("--mi", type=int, default=64, help="gathers per launch, in Mi")
+    q.add_argument("--stride", type=int, default=1, help="word stride of the lanes sharing a line")
     b = sub.add_parser("bfs")
     b.add_argument("--scale", type=int, default=25)
     b.add_argument("--sources", type=int, default=4)
@@ -90,6 +151,8 @@ def main(argv=None) -> int:
     a = ap.parse_args(argv)
     if a.cmd == "copy":
         copy(a.mib, a.repeat)
+    elif a.cmd == "gather":
+        gather(a.table_mib, [int(x) for x in a.share.split(",")], a.mi, stride=a.stride)
     elif a.cmd == "pagerank":
         pagerank(a.scale, a.iterations, a.reorder, a.hot, a.grid, a.tiles, a.sorted, a.batch)
     else:

@@ -240,8 +240,8 @@ def main() -> int:
     ap.add_argument("--bfs-samples-out", default=None, help="save the exhaustive per-level timings (.npz)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
-              "(the multi-GPU sharding is described in DESIGN.md, not run).", file=sys.stderr)
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported: the graph kernels "
+              "here run on a single GPU and no multi-GPU path is built.", file=sys.stderr)
         return 2
 
     from belewitte_amd import device, graph, pagerank
