@@ -257,7 +257,8 @@ def main() -> int:
     fused = pagerank.FusedPageRank.for_graph(g, max_iterations=ITERS, reorder=True)
     log(f"degree-ordered upload + plan in {time.perf_counter() - t0:.1f}s: {fused.info()}")
     shapes = fused.autotune()
-    log(f"launch shapes (ms per 10 iterations): {shapes}; using hot/grid {fused.tuning}")
+    log(f"launch shapes (ms per 10 iterations): {shapes}; using (hot, grid, tiles, window, batch) "
+        f"{fused.tuning}: {fused.kernel_names()}")
 
     # Warm-up, then K timed steps bracketed by device synchronisation.
     for _ in range(args.warmup):
@@ -302,7 +303,7 @@ def main() -> int:
                    "vertices": V, "edges": E, "iterations": ITERS, "parallelism": "single-gpu"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profile(),
-                     "kernel": "pr_pieces_kernel + pr_rows_kernel (one iteration)",
+                     "kernel": fused.kernel_names() + " (one iteration)",
                      "launch_shape": {"hot_vertices": fused.tuning[0], "rows_grid": fused.tuning[1],
                                       "tiles_per_workgroup": fused.tuning[2],
                                       "source_sorted_windows": fused.tuning[3],
