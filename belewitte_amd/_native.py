@@ -185,7 +185,7 @@ _HIP_SIGS = {
     # multi-source BFS
     "bw_msbfs_create": (C.c_int, [C.POINTER(vp), _G, C.c_uint32]),
     "bw_msbfs_destroy": (C.c_int, [vp]),
-    "bw_msbfs_set_tuning": (C.c_int, [vp, C.c_uint32]),
+    "bw_msbfs_set_tuning": (C.c_int, [vp, C.c_uint32, C.c_uint32, C.c_uint32]),
     "bw_msbfs_run": (C.c_int, [vp, u32p, C.c_uint32, vp, C.c_int32, C.c_int32, u64p, u32p, f32p, vp]),
     # variant predictor
     "bw_mlp_params": (C.c_int32, [C.c_int32]),

@@ -26,6 +26,8 @@ from .bfs_native import UNVISITED, BFSEngine, MultiSourceBFS
 
 HUB_CANDIDATES = (1024, 4096, 16384)
 WAVE_CANDIDATES = (8, 16, 32, 64, 128)
+TOPDOWN_DIVISORS = (0, 1, 2, 4, 8, 16)
+HUB_CANDIDATES = (32, 64, 128, 256, 1024)
 
 
 def union(graphs):
@@ -85,17 +87,27 @@ def autotune_engine(eng: BFSEngine, roots, policy: str = "fixed", variant="td", 
     return Tuned(params, table[params], {f"{h}/{w}": v for (h, w), v in table.items()})
 
 
-def autotune_msbfs(ms: MultiSourceBFS, sources, runs: int = 2, waves=WAVE_CANDIDATES + (256, 1 << 30)) -> Tuned:
+def autotune_msbfs(ms: MultiSourceBFS, sources, runs: int = 2, waves=WAVE_CANDIDATES + (256, 1 << 30),
+                   divisors=TOPDOWN_DIVISORS, hubs=HUB_CANDIDATES) -> Tuned:
+    """Coordinate search: the top-down divisor at the default thresholds,
+    then the hub degree, then the wave threshold; returns the fastest
+    (wave, divisor, hub)."""
     table = {}
-    for w in waves:
-        ms.set_tuning(w)
-        best = math.inf
-        for _ in range(runs):
-            best = min(best, ms.run(sources, download=False)[1])
-        table[w] = best
-    w = min(table, key=table.get)
-    ms.set_tuning(w)
-    return Tuned((w,), table[w], {str(k): v for k, v in table.items()})
+
+    def time(w, dv, h):
+        if (w, dv, h) not in table:
+            ms.set_tuning(w, dv, h)
+            best = math.inf
+            for _ in range(runs):
+                best = min(best, ms.run(sources, download=False)[1])
+            table[(w, dv, h)] = best
+        return table[(w, dv, h)]
+
+    dv = min(divisors, key=lambda x: time(32, x, 128))
+    h = min(hubs, key=lambda x: time(32, dv, x))
+    w = min(waves, key=lambda x: time(x, dv, h))
+    ms.set_tuning(w, dv, h)
+    return Tuned((w, dv, h), table[(w, dv, h)], {f"{k[0]}/{k[1]}/{k[2]}": v for k, v in table.items()})
 
 
 def reached_edges(deg: np.ndarray, depths: np.ndarray) -> int:

@@ -211,6 +211,7 @@ class MultiSourceBFS:
         if dg.offset_bytes != 4:
             raise ValueError("multi-source BFS reads uint32 offsets")
         dg.ensure("rev_csr")
+        dg.ensure("csr")
         self.dg = dg
         self.max_levels = int(max_levels or min(dg.vertex_count + 1, 1 << 22))
         p = C.c_void_p()
@@ -219,10 +220,20 @@ class MultiSourceBFS:
         self._depths = None
         self._bits = np.zeros(self.max_levels, dtype=np.uint64)
         self.wave_degree = 32
+        self.topdown_divisor = 4
+        self.hub_degree = 128
 
-    def set_tuning(self, wave_degree: int) -> None:
-        N.check_hip(N.hip_lib().bw_msbfs_set_tuning(self._m, int(wave_degree)))
+    def set_tuning(self, wave_degree: int = 32, topdown_divisor: int = 4, hub_degree: int = 128) -> None:
+        """wave_degree: in-degree above which a whole wave probes a vertex
+        (bottom-up levels); hub_degree: in-degree above which a vertex's
+        in-edges are split into chunks of 512, one wave each (bottom-up);
+        a level runs top-down when its frontier's out-edges times
+        topdown_divisor are at most E (0: never).  Depths depend on none."""
+        N.check_hip(N.hip_lib().bw_msbfs_set_tuning(self._m, int(wave_degree), int(topdown_divisor),
+                                                    int(hub_degree)))
         self.wave_degree = int(wave_degree)
+        self.topdown_divisor = int(topdown_divisor)
+        self.hub_degree = int(hub_degree)
 
     def run(self, sources, levels_per_sync: int = 8, use_graph: bool = True, download: bool = True, stream=None):
         """Returns (depths int32[n][V] or None, total_ms, level_bits)."""

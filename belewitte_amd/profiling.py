@@ -10,6 +10,9 @@
         prints lanes/s and distinct lines/s per sharing factor.
     python -m belewitte_amd.profiling pagerank --scale 22 --iterations 5 [--reorder] [--grid N --hot H --tiles T] [--sorted W --batch B]
         A few fused PageRank iterations on the benchmark graph.
+    python -m belewitte_amd.profiling msbfs --scale 20 --sources 64 [--wave-degree W --topdown-divisor D --hub-degree H]
+        One multi-source search (64 sources) on the directed R-MAT of the
+        batched BFS workload.
     python -m belewitte_amd.profiling bfs --scale 25 --sources 4 [--policy learned|td|bu|bu-wave]
         BFS engine searches on the undirected R-MAT of the learned-switch
         workload (the shipped model for the learned policy).
@@ -106,6 +109,23 @@ def pagerank(scale: int, iterations: int, reorder: bool, hot: int = 0, grid: int
     print(f"pagerank: V={g.vertex_count} E={g.edge_count} {iterations} iterations {ms:.3f} ms", flush=True)
 
 
+def msbfs(scale: int, sources: int, wave_degree: int, topdown_divisor: int, hub_degree: int) -> None:
+    from . import bfs_native, device, graph
+    device.require_gpu()
+    device.set_device(0)
+    g = graph.rmat(scale, 16, seed=6)
+    dg = device.DeviceGraph(g)
+    src = graph.random_sources(g, sources, seed=9)
+    ms = bfs_native.MultiSourceBFS(dg)
+    ms.set_tuning(wave_degree, topdown_divisor, hub_degree)
+    ms.run(src, download=False)
+    _, t, bits = ms.run(src, download=False)
+    print(f"msbfs: V={g.vertex_count} E={g.edge_count} {len(src)} sources {t:.3f} ms, {len(bits)} levels, "
+          f"new (source, vertex) pairs per level {bits.tolist()}", flush=True)
+    ms.close()
+    dg.free()
+
+
 def bfs(scale: int, sources: int, policy: str) -> None:
     from . import bfs_native, device, graph, switch
     device.require_gpu()
@@ -144,6 +164,12 @@ def main(argv=None) -> int:
     q.add_argument("--share", default="1,2,4,8,16,32,64", help="lanes per line, comma separated")
     q.add_argument("--mi", type=int, default=64, help="gathers per launch, in Mi")
     q.add_argument("--stride", type=int, default=1, help="word stride of the lanes sharing a line")
+    m = sub.add_parser("msbfs")
+    m.add_argument("--scale", type=int, default=20)
+    m.add_argument("--sources", type=int, default=64)
+    m.add_argument("--wave-degree", type=int, default=32)
+    m.add_argument("--topdown-divisor", type=int, default=4)
+    m.add_argument("--hub-degree", type=int, default=128)
     b = sub.add_parser("bfs")
     b.add_argument("--scale", type=int, default=25)
     b.add_argument("--sources", type=int, default=4)
@@ -153,6 +179,8 @@ def main(argv=None) -> int:
         copy(a.mib, a.repeat)
     elif a.cmd == "gather":
         gather(a.table_mib, [int(x) for x in a.share.split(",")], a.mi, stride=a.stride)
+    elif a.cmd == "msbfs":
+        msbfs(a.scale, a.sources, a.wave_degree, a.topdown_divisor, a.hub_degree)
     elif a.cmd == "pagerank":
         pagerank(a.scale, a.iterations, a.reorder, a.hot, a.grid, a.tiles, a.sorted, a.batch)
     else:

@@ -203,7 +203,9 @@ def bfs_batched() -> dict:
     agree = all(np.array_equal(eng.run(int(src[i]), "fixed", "bu-wave").levels, depths[i]) for i in (0, 31, 63))
     eng.close()
     out["sources_64_rmat_s20"] = {"ms": round(best, 3), "gteps": round(edges / (best / 1e3) / 1e9, 2),
-                                  "wave_degree": tuned.params[0], "tuning_ms": tuned.table,
+                                  "wave_degree": tuned.params[0], "topdown_divisor": tuned.params[1],
+                                  "hub_degree": tuned.params[2],
+                                  "tuning_ms": tuned.table,
                                   "reached_edges": edges, "spot_check_vs_single_source": bool(agree)}
     ms.close()
     dg.free()

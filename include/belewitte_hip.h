@@ -550,16 +550,26 @@ int bw_bfs_engine_run_multi(bw_bfs_engine *engine, const uint32_t *roots,
 /* Up to 64 BFS searches on one graph sharing every pass over it (64-bit
  * visit/frontier masks per vertex, bottom-up levels with early exit, the
  * level loop on the device).  Search i computes the depths of
- * bfs/bfs.cpp:43-118 from sources[i].  Needs the reverse CSR with uint32
- * offsets (bw_graph_dev.rev_offsets / rev_cols). */
+ * bfs/bfs.cpp:43-118 from sources[i].  Needs the forward and reverse CSR
+ * with uint32 offsets (offsets / cols for top-down levels, rev_offsets /
+ * rev_cols for bottom-up ones). */
 typedef struct bw_msbfs bw_msbfs;
 
 #define BW_MSBFS_DEFAULT_WAVE_DEGREE 32
+#define BW_MSBFS_DEFAULT_TOPDOWN_DIVISOR 4
+#define BW_MSBFS_DEFAULT_HUB_DEGREE 128
 
 int bw_msbfs_create(bw_msbfs **ms, const bw_graph_dev *g, uint32_t max_levels);
 int bw_msbfs_destroy(bw_msbfs *ms);
-/* In-degree above which a whole wave probes a vertex's in-neighbours. */
-int bw_msbfs_set_tuning(bw_msbfs *ms, uint32_t wave_degree);
+/* wave_degree: in-degree above which a whole wave probes a vertex's
+ * in-neighbours (bottom-up levels).  hub_degree (>= 16): in-degree above
+ * which a vertex's in-edges are split into chunks of 512, one wave each
+ * (bottom-up levels; changing it rebuilds the chunk list).
+ * topdown_divisor (0..1024): a level runs top-down when the out-edges of
+ * its frontier times the divisor are at most E, bottom-up otherwise; 0
+ * keeps every level bottom-up.  Depths do not depend on any of them. */
+int bw_msbfs_set_tuning(bw_msbfs *ms, uint32_t wave_degree,
+                        uint32_t topdown_divisor, uint32_t hub_degree);
 /* depths: device int32[n][V], row i = depths from sources[i] (host array,
  * 1 <= n <= 64).  Levels are launched in batches of levels_per_sync
  * (replayed as a captured hipGraph if use_graph).  Optional host outputs:

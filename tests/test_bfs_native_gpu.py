@@ -264,10 +264,16 @@ def test_rejects_bad_arguments(cases):
 
 @pytest.mark.parametrize("nsrc", [1, 17, 64])
 @pytest.mark.parametrize("use_graph", [False, True])
-def test_multi_source_bit_exact(cases, nsrc, use_graph):
+@pytest.mark.parametrize("td", [0, 1, 4])
+def test_multi_source_bit_exact(cases, nsrc, use_graph, td):
+    """td 0: every level bottom-up; 1: every level top-down (the frontier's
+    out-edges never exceed E); 4: the direction changes between levels.
+    Hub degree 16: every vertex above 16 in-edges goes through the
+    edge-parallel hub chunks in the bottom-up levels."""
     rng = np.random.default_rng(nsrc)
     for g, csr, dg, eng, roots in cases:
         ms = bfs_native.MultiSourceBFS(dg)
+        ms.set_tuning(32, td, 16 if nsrc == 17 else 128)
         try:
             src = rng.integers(0, g.vertex_count, nsrc)
             src[0] = roots[0]
@@ -285,14 +291,19 @@ def test_multi_source_bit_exact(cases, nsrc, use_graph):
             ms.close()
 
 
-@pytest.mark.parametrize("wave", [0, 4, 1 << 20])
-def test_multi_source_tuning(cases, wave):
+@pytest.mark.parametrize("wave,td,hub", [(0, 0, 128), (4, 0, 16), (1 << 20, 0, 1 << 20), (4, 2, 64), (0, 1024, 16),
+                                         (64, 0, 16)])
+def test_multi_source_tuning(cases, wave, td, hub):
     g, csr, dg, eng, roots = cases[0]
     ms = bfs_native.MultiSourceBFS(dg)
-    ms.set_tuning(wave)
+    ms.set_tuning(wave, td, hub)
     src = np.arange(0, g.vertex_count, g.vertex_count // 64)[:64]
     depths, _, _ = ms.run(src)
     for i, s in enumerate(src):
         want, _ = O.bfs_queue(csr, int(s))
-        assert np.array_equal(depths[i], want), (wave, i)
+        assert np.array_equal(depths[i], want), (wave, td, hub, i)
+    with pytest.raises(N.BelewitteError):
+        ms.set_tuning(wave, 1025, hub)
+    with pytest.raises(N.BelewitteError):
+        ms.set_tuning(wave, td, 15)
     ms.close()
