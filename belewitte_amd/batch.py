@@ -10,9 +10,13 @@ captured hipGraph batches (no host round trip per level):
   each component gets exactly its own BFS depths.
 
 The autotuner times whole batch runs over a grid of the kernels'
-thread/wave/workgroup thresholds (hub out-degree for workgroup-sliced
-top-down expansion, in-degree for wave-wide bottom-up probes) and keeps
-the fastest; the thresholds never change the depths, only the time.
+thread/wave/workgroup thresholds and keeps the fastest:
+- the engine: hub out-degree for workgroup-sliced top-down expansion and
+  in-degree for wave-wide bottom-up probes;
+- the multi-source search: the divisor that picks top-down levels, the
+  in-degree above which a vertex's in-edges are split into wave-sized
+  chunks, and the in-degree for wave-wide probes.
+The thresholds never change the depths, only the time.
 """
 from __future__ import annotations
 
@@ -27,7 +31,7 @@ from .bfs_native import UNVISITED, BFSEngine, MultiSourceBFS
 HUB_CANDIDATES = (1024, 4096, 16384)
 WAVE_CANDIDATES = (8, 16, 32, 64, 128)
 TOPDOWN_DIVISORS = (0, 1, 2, 4, 8, 16)
-HUB_CANDIDATES = (32, 64, 128, 256, 1024)
+MSBFS_HUB_CANDIDATES = (32, 64, 128, 256, 1024)
 
 
 def union(graphs):
@@ -88,7 +92,7 @@ def autotune_engine(eng: BFSEngine, roots, policy: str = "fixed", variant="td", 
 
 
 def autotune_msbfs(ms: MultiSourceBFS, sources, runs: int = 2, waves=WAVE_CANDIDATES + (256, 1 << 30),
-                   divisors=TOPDOWN_DIVISORS, hubs=HUB_CANDIDATES) -> Tuned:
+                   divisors=TOPDOWN_DIVISORS, hubs=MSBFS_HUB_CANDIDATES) -> Tuned:
     """Coordinate search: the top-down divisor at the default thresholds,
     then the hub degree, then the wave threshold; returns the fastest
     (wave, divisor, hub)."""

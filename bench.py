@@ -185,9 +185,10 @@ def bfs_batched() -> dict:
     """Batch 1: 64 sources on a directed R-MAT scale 20, edge factor 16
     (seed 6) in one multi-source search; batch 2: 256 directed R-MAT graphs
     of 2^10..2^16 vertices (log-uniform) and edge factors 4..32 searched as
-    one disjoint union from their max-out-degree vertices.  Both autotuned,
+    one disjoint union from their max-out-degree vertices with the learned
+    per-level switch (the best fixed variant beside it).  Both autotuned,
     both one device-driven level loop in hipGraph batches."""
-    from belewitte_amd import batch, bfs_native, device, graph
+    from belewitte_amd import batch, bfs_native, device, graph, switch
     out = {}
     g = graph.rmat(20, 16, seed=6)
     dg = device.DeviceGraph(g)
@@ -215,12 +216,19 @@ def bfs_batched() -> dict:
     roots = [base[i] + gi.max_out_degree_vertex() for i, gi in enumerate(graphs)]
     dg = device.DeviceGraph(u)
     eng = bfs_native.BFSEngine(dg)
-    tuned = batch.autotune_engine(eng, roots, "fixed", "td")
-    best = min(eng.run(roots, "fixed", "td", download=False).total_ms for _ in range(5))
-    r = eng.run(roots, "fixed", "td")
+    eng.set_model(switch.Model.load(switch.MODEL_PATH).w)
+    # The best fixed variant, for contrast with the learned per-level switch.
+    fixed = {v: min(eng.run(roots, "fixed", v, download=False).total_ms for _ in range(3))
+             for v in bfs_native.VARIANTS}
+    tuned = batch.autotune_engine(eng, roots, "learned", "td")
+    best = min(eng.run(roots, "learned", "td", download=False).total_ms for _ in range(5))
+    r = eng.run(roots, "learned", "td")
     deg = np.diff(u.offsets()).astype(np.int64)
     edges = batch.reached_edges(deg, r.levels)
     out["graphs_256_rmat"] = {"ms": round(best, 3), "gteps": round(edges / (best / 1e3) / 1e9, 2),
+                              "policy": "learned per-level switch (shipped model)",
+                              "choices": r.choices.tolist(),
+                              "fixed_variant_ms": {k: round(v, 3) for k, v in fixed.items()},
                               "vertices": u.vertex_count, "edges": u.edge_count, "reached_edges": edges,
                               "levels": r.num_levels, "tuned_hub_wave": list(tuned.params),
                               "tuning_ms": tuned.table}
