@@ -42,6 +42,8 @@ namespace {
 constexpr int kBlock = 256;
 constexpr uint32_t kLdsQueue = 2048;   // staged discoveries per workgroup
 constexpr uint32_t kHubCap = 4096;     // hubs per level (more go to the normal queue)
+constexpr uint32_t kParentScan = 256;  // in-edges scanned for a vertex's likely parent
+constexpr uint32_t kNoParent = 0xFFFFFFFFu;
 constexpr int kFeatures = BW_BFS_FEATURES;
 constexpr int kHidden = 32;
 constexpr int kMaxOut = 16;
@@ -84,6 +86,8 @@ struct Buffers {
     uint64_t *hub_prefix;  // kHubCap + 1
     uint32_t *bits[3];     // frontier bitmaps, level d reads bits[d % 3]
     uint32_t *vis;         // visited bitmap
+    const uint32_t *parent;  // per vertex: its likely parent (in-neighbour of largest out-degree among
+                             // its first kParentScan in-edges), kNoParent without in-edges
     BfsState *st;
     uint64_t *part;        // per workgroup: out-degree sum, (double) sum of squares
     uint32_t *counts;      // per level: discovered vertices
@@ -269,8 +273,12 @@ __device__ void level_bu(const Graph &g, const Buffers &b, uint32_t d, Emitter &
         const uint64_t v64 = base + threadIdx.x;
         const uint32_t v = static_cast<uint32_t>(v64);
         const bool unvisited = v64 < V && !test_bit(b.vis, v);
+        // The likely parent first: one coalesced load and one probe resolve
+        // most vertices of a big level; the in-edges are scanned otherwise.
+        const uint32_t pv = unvisited ? b.parent[v] : kNoParent;
+        const bool quick = pv != kNoParent && test_bit(front, pv);
         uint32_t s = 0, e = 0;
-        if (unvisited) {
+        if (unvisited && !quick) {
             s = g.roff[v];
             e = g.roff[v + 1];
         }
@@ -278,7 +286,7 @@ __device__ void level_bu(const Graph &g, const Buffers &b, uint32_t d, Emitter &
         // probes per step; BU_WAVE stops a lane after wave_degree edges and
         // hands the rest of a still-unresolved row to the whole wave.
         const uint32_t lim = wave && e - s > b.wave_degree ? s + b.wave_degree : e;
-        bool discovered = false;
+        bool discovered = quick;
         for (uint32_t j = s; j < lim; j += 4) {
             bool hit = false;
 #pragma unroll
@@ -633,6 +641,28 @@ __global__ __launch_bounds__(kBlock) void bfs_engine_init(Graph g, Buffers b, Po
 __global__ void set_choice(BfsState *st, int32_t c) { st->choice = c; }
 __global__ void copy_state(BfsState *dst, const BfsState *src) { *dst = *src; }
 
+// Likely parent of each vertex: among its first kParentScan in-edges, the
+// in-neighbour of largest out-degree (the first one on ties), which a
+// bottom-up level probes before the in-edges.  Only an order of probes:
+// a vertex is discovered iff some in-neighbour is in the frontier.
+__global__ __launch_bounds__(kBlock) void likely_parent_kernel(Graph g, uint32_t *parent)
+{
+    for (uint64_t v = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x; v < g.V;
+         v += static_cast<uint64_t>(gridDim.x) * kBlock) {
+        const uint32_t s = g.roff[v], e = min(g.roff[v + 1], s + kParentScan);
+        uint32_t best = kNoParent, bestDeg = 0;
+        for (uint32_t j = s; j < e; j++) {
+            const uint32_t u = g.rcol[j];
+            const uint32_t du = out_degree(g, u);
+            if (best == kNoParent || du > bestDeg) {
+                best = u;
+                bestDeg = du;
+            }
+        }
+        parent[v] = best;
+    }
+}
+
 // Bitmap of the vertices without in-edges: no search can reach them except
 // as a root, so each search starts with them marked visited and bottom-up
 // levels never scan them.  A wave covers 64 consecutive vertices, i.e. two
@@ -673,6 +703,7 @@ struct bw_bfs_engine {
     uint32_t *roots = nullptr;
     uint32_t roots_cap = 0;
     uint32_t *no_in = nullptr;  // vertices without in-edges (bitmap)
+    uint32_t *parent = nullptr;  // likely parent per vertex
     hipStream_t own_stream = nullptr;
     hipGraphExec_t exec = nullptr;
     int32_t exec_key[4] = {-1, -1, -1, -1};  // policy, fixed, schedule_len, levels per batch
@@ -689,7 +720,7 @@ struct bw_bfs_engine {
         drop_exec();
         void *bufs[] = {own_levels, b.q[0], b.q[1], b.hubq[0], b.hubq[1], b.hub_prefix, b.bits[0], b.bits[1],
                         b.bits[2], b.vis, b.st, b.part, b.counts, b.times, b.choices, snap_levels, snap_q, snap_hub,
-                        snap_bits, snap_prefix, snap_st, mlp, schedule, roots, no_in};
+                        snap_bits, snap_prefix, snap_st, mlp, schedule, roots, no_in, parent};
         for (void *p : bufs)
             if (p) (void)hipFree(p);
         if (own_stream) (void)hipStreamDestroy(own_stream);
@@ -772,6 +803,7 @@ extern "C" int bw_bfs_engine_create(bw_bfs_engine **out, const bw_graph_dev *g, 
     for (auto &bm : e->b.bits) alloc(&bm, sizeof(uint32_t) * words);
     alloc(&e->b.vis, sizeof(uint32_t) * words);
     alloc(&e->no_in, sizeof(uint32_t) * words);
+    alloc(&e->parent, sizeof(uint32_t) * V);
     alloc(&e->b.st, sizeof(BfsState));
     alloc(&e->b.part, sizeof(uint64_t) * 2 * e->grid);
     alloc(&e->b.counts, sizeof(uint32_t) * max_levels);
@@ -786,7 +818,9 @@ extern "C" int bw_bfs_engine_create(bw_bfs_engine **out, const bw_graph_dev *g, 
         delete e;
         return fail(BW_HIP_ERR_NOMEM, "device allocation of the BFS engine failed");
     }
+    e->b.parent = e->parent;
     hipLaunchKernelGGL(no_in_edges_kernel, dim3(e->grid), dim3(kBlock), 0, nullptr, e->g, e->no_in);
+    hipLaunchKernelGGL(likely_parent_kernel, dim3(e->grid), dim3(kBlock), 0, nullptr, e->g, e->parent);
     if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess) {
         delete e;
         return fail(BW_HIP_ERR_RUNTIME, "building the in-edge bitmap failed");

@@ -135,13 +135,20 @@ def bfs(scale: int, sources: int, policy: str) -> None:
     eng = bfs_native.BFSEngine(dg)
     if policy == "learned":
         eng.set_model(switch.Model.load(switch.MODEL_PATH).w)
-    for s in graph.random_sources(g, sources, seed=8):
+    def one(s):
         if policy == "learned":
-            r = eng.run(s, "learned", "td", download=False)
-        else:
-            r = eng.run(s, "fixed", policy, download=False)
+            return eng.run(s, "learned", "td", download=False)
+        return eng.run(s, "fixed", policy, download=False)
+
+    src = graph.random_sources(g, sources, seed=8)
+    one(src[0])  # warm-up: builds the captured level batch
+    total = 0.0
+    for s in src:
+        r = one(s)
+        total += r.total_ms
         print(f"bfs {policy}: source {s} {r.num_levels} levels {r.total_ms:.3f} ms "
               f"variants {r.choices.tolist()}", flush=True)
+    print(f"bfs {policy}: {len(src)} sources {total:.3f} ms", flush=True)
 
 
 def main(argv=None) -> int:
