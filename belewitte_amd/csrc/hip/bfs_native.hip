@@ -122,19 +122,17 @@ struct Emitter {
     double sq = 0.0;
 
     // Called by the lanes that discovered a vertex (any subset of a wave).
-    // mark: set w's bits in the next-frontier and visited bitmaps here
-    // (top-down, where any workgroup may discover any vertex); bottom-up
-    // levels set whole words per wave instead (see level_bu).
+    // mark: set w's bit in the next-frontier bitmap here (top-down, where
+    // any workgroup may discover any vertex; try_discover has set its
+    // visited bit); bottom-up levels set whole words per wave instead (see
+    // level_bu).
     __device__ __forceinline__ void emit(const Graph &g, const Buffers &b, uint32_t d, uint32_t w, bool mark)
     {
         const uint32_t nq = (d & 1) ^ 1;
         const uint32_t dw = out_degree(g, w);
         outsum += dw;
         sq += static_cast<double>(dw) * static_cast<double>(dw);
-        if (mark) {
-            atomicOr(&b.bits[(d + 1) % 3][w >> 5], 1u << (w & 31));
-            atomicOr(&b.vis[w >> 5], 1u << (w & 31));
-        }
+        if (mark) atomicOr(&b.bits[(d + 1) % 3][w >> 5], 1u << (w & 31));
         if (dw > b.hub_degree) {
             // Slots are handed out by a counter that only grows, so every
             // slot below min(count, kHubCap) is written exactly once.
@@ -179,9 +177,16 @@ struct Emitter {
     }
 };
 
-__device__ __forceinline__ bool try_discover(int32_t *levels, uint32_t w, int32_t nd)
+// Top-down discovery of w: the thread whose fetch-or sets w's visited bit
+// owns w and records its depth.  Edges to visited vertices cost one read of
+// the visited bitmap (V bits, far more cacheable than the depth array).
+__device__ __forceinline__ bool try_discover(const Buffers &b, uint32_t w, int32_t nd)
 {
-    return levels[w] == kUnvisited && atomicCAS(&levels[w], kUnvisited, nd) == kUnvisited;
+    const uint32_t bit = 1u << (w & 31);
+    if (b.vis[w >> 5] & bit) return false;
+    if (atomicOr(&b.vis[w >> 5], bit) & bit) return false;
+    b.levels[w] = nd;
+    return true;
 }
 
 // ------------------------------------------------------------- variants
@@ -210,7 +215,7 @@ __device__ void level_td(const Graph &g, const Buffers &b, uint32_t d, Emitter &
                 }
                 const uint32_t u = b.hubq[cur][lo];
                 const uint32_t w = g.col[g.off[u] + static_cast<uint32_t>(k - b.hub_prefix[lo])];
-                if (try_discover(b.levels, w, nd)) em.emit(g, b, d, w, true);
+                if (try_discover(b, w, nd)) em.emit(g, b, d, w, true);
             }
             em.flush(b, d, kBlock);
         }
@@ -254,7 +259,7 @@ __device__ void level_td(const Graph &g, const Buffers &b, uint32_t d, Emitter &
                 }
                 const uint64_t before = lo ? sscan[lo - 1] : 0;
                 const uint32_t w = g.col[sstart[lo] + static_cast<uint32_t>(k - before)];
-                if (try_discover(b.levels, w, nd)) em.emit(g, b, d, w, true);
+                if (try_discover(b, w, nd)) em.emit(g, b, d, w, true);
             }
             em.flush(b, d, kBlock);
         }
