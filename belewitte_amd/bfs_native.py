@@ -25,7 +25,7 @@ FEATURE_NAMES = ("log2_frontier_fraction", "log2_frontier_edge_fraction", "log2_
                  "log2_mean_degree", "log2_std_degree", "prev_variant", "log2_prev_time_ns")
 UNVISITED = 2147483647
 DEFAULT_HUB_DEGREE = 4096
-DEFAULT_WAVE_DEGREE = 32
+DEFAULT_WAVE_DEGREE = 8
 
 
 def variant_id(v) -> int:

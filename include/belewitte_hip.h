@@ -478,7 +478,7 @@ int bw_bfs_engine_levels(const bw_bfs_engine *engine, int32_t **levels);
  * over all workgroups (>= 256), and in-degree above which BU_WAVE probes
  * with a whole wave instead of one thread. */
 #define BW_BFS_DEFAULT_HUB_DEGREE 4096
-#define BW_BFS_DEFAULT_WAVE_DEGREE 32
+#define BW_BFS_DEFAULT_WAVE_DEGREE 8
 int bw_bfs_engine_set_tuning(bw_bfs_engine *engine, uint32_t hub_degree,
                              uint32_t wave_degree);
 
