@@ -531,18 +531,17 @@ __global__ __launch_bounds__(kBlock) void bfs_engine_epilogue(Graph g, Buffers b
             mine[k] = h < nhub ? out_degree(g, b.hubq[nq][h]) : 0;
             local += mine[k];
         }
-        hscan[tid] = local;
-        __syncthreads();
-        if (tid == 0) {
-            uint64_t run = 0;
-            for (int i = 0; i < kBlock; i++) {
-                const uint64_t x = hscan[i];
-                hscan[i] = run;
-                run += x;
-            }
+        // Exclusive scan of the threads' sums: within each wave by
+        // shuffles, then the preceding waves' totals.
+        uint64_t incl = local;
+        for (int o = 1; o < kWave; o <<= 1) {
+            const uint64_t y = __shfl_up(incl, static_cast<unsigned>(o), kWave);
+            if (lane >= o) incl += y;
         }
+        if (lane == kWave - 1) hscan[wid] = incl;
         __syncthreads();
-        uint64_t run = hscan[tid];
+        uint64_t run = incl - local;
+        for (int w = 0; w < wid; w++) run += hscan[w];
         for (uint32_t k = 0; k < kPer; k++) {
             const uint32_t h = tid * kPer + k;
             if (h <= nhub) b.hub_prefix[h] = run;
