@@ -9,8 +9,10 @@
 //  - bottom-up: every vertex not yet reached by all searches ORs the
 //    frontier masks of its in-neighbours, stopping as soon as that covers
 //    every search still missing it.  One thread probes a vertex with few
-//    in-edges, a whole wave (64 in-edges per step) one above the tunable
-//    wave threshold;
+//    in-edges, a whole wave (256 in-edges per round trip) one above the
+//    tunable wave threshold; a vertex above the tunable hub degree is cut
+//    into chunks of in-edges, one wave each, that OR into next[] with
+//    atomics (msbfs_bu_hubs);
 //  - top-down: the frontier vertices are queued with their out-edge ranges
 //    reserved in one counter, then every frontier edge u -> w ORs u's mask
 //    into next[w] (edge-parallel, so a hub's edges spread over the grid).
