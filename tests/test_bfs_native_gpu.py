@@ -36,8 +36,21 @@ def _many_hubs():
     return graph.from_edges(V, np.concatenate(src), np.concatenate(dst))
 
 
+def _permuted_rmat():
+    """An R-MAT with randomly relabelled vertices: the hubs are no longer the
+    lowest ids, so a vertex's likely parent (its in-neighbour of largest
+    out-degree) sits anywhere in its in-list."""
+    g = graph.rmat(12, 16, seed=23)
+    perm = np.random.default_rng(5).permutation(g.vertex_count).astype(np.uint32)
+    s, d = g.edges()
+    p = graph.from_edges(g.vertex_count, perm[s], perm[d])
+    p.name = "rmat-12-permuted"
+    return p
+
+
 def _graphs():
     yield graph.rmat(12, 16, seed=21)
+    yield _permuted_rmat()
     yield graph.rmat(11, 8, seed=22, undirected=True)
     yield graph.erdos_renyi(5000, 20000, seed=7, undirected=True)
     yield graph.grid2d(64, 48, rewire=0.01, seed=9)
