@@ -2,7 +2,7 @@
 
 The reference switches BFS implementations per level through a model
 looked up in SwitchImplementation (ImplementationTemplate.hpp:545-908)
-and judges it against the best non-switching implementation and the
+and compares it with the best non-switching implementation and the
 per-level optimum (benchmark-analysis/src/StepAggregate.hs:95-119,
 benchmark-analysis/model-src/Evaluate.hs:229-230).  Here:
 
