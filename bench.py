@@ -16,8 +16,9 @@ oracle, the per-iteration rate of each of the reference's 10 variants, and
 the BFS figures: the learned per-level switch against the best fixed
 variant and the per-level oracle on an undirected R-MAT scale 25 (GTEPS,
 with a CPU baseline), the batched searches (64 sources on R-MAT scale 20;
-256 small R-MAT graphs), and the reference's vertex-push on R-MAT scale 16
-(GPU and CPU).
+256 small R-MAT graphs), the learned switch and the fixed variants on the
+switch's uniform (Erdos-Renyi 2^24) and grid (4096 x 4096) training graphs,
+and the reference's vertex-push on R-MAT scale 16 (GPU and CPU).
 """
 from __future__ import annotations
 
@@ -106,6 +107,52 @@ def bfs_learned_switch(scale: int, sources: int, samples_out=None) -> dict:
     eng.close()
     dg.free()
     return res
+
+
+def bfs_training_graphs(sources: int = 16) -> dict:
+    """BFS on the two contrasting graphs of the learned switch's training
+    set: an undirected Erdos-Renyi graph (2^24 vertices, average degree 16,
+    seed 3) and a 4096 x 4096 4-neighbour grid with 1% of its edges rewired
+    (seed 4).  For each, executed runs from `sources` random sources (seed
+    10) of the learned switch and of every fixed native variant (device
+    time summed over the sources, GTEPS in the undirected convention); the
+    learned runs' depths are compared with the fixed top-down runs' on two
+    sources."""
+    from belewitte_amd import bfs_native, device, graph, switch
+    model = switch.Model.load(switch.MODEL_PATH)
+    out = {}
+    for g in (graph.erdos_renyi(1 << 24, 134217728, seed=3, undirected=True),
+              graph.grid2d(4096, 4096, rewire=0.01, seed=4)):
+        t0 = time.perf_counter()
+        dg = device.DeviceGraph(g)
+        eng = bfs_native.BFSEngine(dg)
+        eng.set_model(model.w)
+        src = [int(x) for x in graph.random_sources(g, sources, seed=10)]
+        edges, levels, same = 0, 0, True
+        for i, s in enumerate(src):
+            r = eng.run(s, "learned", "td")
+            edges += bfs_native.teps_edges(g, r.levels)
+            levels += r.num_levels
+            if i < 2:
+                same &= bool(np.array_equal(r.levels, eng.run(s, "fixed", "td").levels))
+        ms = {}
+        for pol, var in [("learned", "td")] + [("fixed", v) for v in bfs_native.VARIANTS]:
+            eng.run(src[0], pol, var, download=False)
+            ms["learned" if pol == "learned" else var] = sum(
+                eng.run(s, pol, var, download=False).total_ms for s in src)
+        fixed = {k: v for k, v in ms.items() if k != "learned"}
+        best = min(fixed, key=fixed.get)
+        out[g.name] = {"vertices": g.vertex_count, "stored_edges": g.edge_count, "sources": len(src),
+                       "levels": levels, "teps_edges": edges, "executed_ms": {k: round(v, 3) for k, v in ms.items()},
+                       "gteps": {"learned": round(edges / (ms["learned"] / 1e3) / 1e9, 2),
+                                 "best_fixed": round(edges / (fixed[best] / 1e3) / 1e9, 2)},
+                       "best_fixed_variant": best,
+                       "speedup_vs_best_fixed": round(fixed[best] / ms["learned"], 3),
+                       "depths_equal_to_fixed_td": same}
+        log(f"bfs {g.name}: {out[g.name]['executed_ms']} ({time.perf_counter() - t0:.1f}s)")
+        eng.close()
+        dg.free()
+    return out
 
 
 def bfs_reference_vertex_push(cpu: bool) -> dict:
@@ -245,6 +292,8 @@ def main() -> int:
     ap.add_argument("--no-variants", action="store_true", help="skip the per-variant table")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-bfs", action="store_true", help="skip the BFS figures")
+    ap.add_argument("--no-training-graphs", action="store_true",
+                    help="skip the BFS runs on the Erdos-Renyi and grid graphs")
     ap.add_argument("--bfs-scale", type=int, default=25, help="R-MAT scale of the learned-switch BFS")
     ap.add_argument("--bfs-sources", type=int, default=16)
     ap.add_argument("--bfs-samples-out", default=None, help="save the exhaustive per-level timings (.npz)")
@@ -368,6 +417,8 @@ def main() -> int:
             out["bfs_learned_switch"]["cpu_baseline"] = bfs_cpu_baseline(args.bfs_scale)
             log(f"bfs cpu baseline {out['bfs_learned_switch']['cpu_baseline']}")
         out["bfs_batched"] = bfs_batched()
+        if not args.no_training_graphs:
+            out["bfs_uniform_and_grid"] = bfs_training_graphs()
         out["bfs_reference_vertex_push_rmat_s16"] = bfs_reference_vertex_push(not args.no_cpu)
 
     print(json.dumps(out), flush=True)
