@@ -68,6 +68,8 @@ struct BfsState {
     uint64_t frontier_edges;    // out-degree sum of the frontier
     double frontier_sq;         // sum of squared out-degrees of the frontier
     uint64_t t_step;            // wall clock at the start of the level's step kernel
+    uint32_t fill_levels;       // the previous level left its depths to this one (dense top-down)
+    uint32_t pad_;
 };
 
 struct Graph {
@@ -178,15 +180,26 @@ struct Emitter {
 };
 
 // Top-down discovery of w: the thread whose fetch-or sets w's visited bit
-// owns w and records its depth.  Edges to visited vertices cost one read of
-// the visited bitmap (V bits, far more cacheable than the depth array).
-__device__ __forceinline__ bool try_discover(const Buffers &b, uint32_t w, int32_t nd)
+// owns w and records its depth (unless the level defers its depths).
+// Edges to visited vertices cost one read of the visited bitmap (V bits,
+// far more cacheable than the depth array).
+__device__ __forceinline__ bool try_discover(const Buffers &b, uint32_t w, int32_t nd, bool write_depth)
 {
     const uint32_t bit = 1u << (w & 31);
     if (b.vis[w >> 5] & bit) return false;
     if (atomicOr(&b.vis[w >> 5], bit) & bit) return false;
-    b.levels[w] = nd;
+    if (write_depth) b.levels[w] = nd;
     return true;
+}
+
+// A dense top-down level (frontier out-edges above V/16) leaves the depths
+// of its discoveries to the next level, which writes depth d + 1 for every
+// vertex of its frontier bitmap with coalesced stores instead of one
+// scattered store per discovery.  Never at the last levels the limit
+// allows, so the next level always runs.
+__device__ __forceinline__ bool defer_depths(const BfsState &st, const Buffers &b, uint64_t V)
+{
+    return st.choice == TD && st.frontier_edges * 16 > V && st.depth + 2 < b.max_levels;
 }
 
 // ------------------------------------------------------------- variants
@@ -198,6 +211,7 @@ __device__ void level_td(const Graph &g, const Buffers &b, uint32_t d, Emitter &
     const uint32_t cur = d & 1;
     const BfsState &st = *b.st;
     const uint32_t tid = threadIdx.x;
+    const bool wdepth = !defer_depths(st, b, g.V);
 
     // Hub edges, sliced over all workgroups 256 edges at a time.
     const uint32_t nh = st.hubcount;
@@ -215,7 +229,7 @@ __device__ void level_td(const Graph &g, const Buffers &b, uint32_t d, Emitter &
                 }
                 const uint32_t u = b.hubq[cur][lo];
                 const uint32_t w = g.col[g.off[u] + static_cast<uint32_t>(k - b.hub_prefix[lo])];
-                if (try_discover(b, w, nd)) em.emit(g, b, d, w, true);
+                if (try_discover(b, w, nd, wdepth)) em.emit(g, b, d, w, true);
             }
             em.flush(b, d, kBlock);
         }
@@ -259,7 +273,7 @@ __device__ void level_td(const Graph &g, const Buffers &b, uint32_t d, Emitter &
                 }
                 const uint64_t before = lo ? sscan[lo - 1] : 0;
                 const uint32_t w = g.col[sstart[lo] + static_cast<uint32_t>(k - before)];
-                if (try_discover(b, w, nd)) em.emit(g, b, d, w, true);
+                if (try_discover(b, w, nd, wdepth)) em.emit(g, b, d, w, true);
             }
             em.flush(b, d, kBlock);
         }
@@ -267,7 +281,7 @@ __device__ void level_td(const Graph &g, const Buffers &b, uint32_t d, Emitter &
     }
 }
 
-__device__ void level_bu(const Graph &g, const Buffers &b, uint32_t d, Emitter &em, bool wave)
+__device__ void level_bu(const Graph &g, const Buffers &b, uint32_t d, Emitter &em, bool wave, bool fill)
 {
     const int32_t nd = static_cast<int32_t>(d) + 1;
     const uint32_t *front = b.bits[d % 3];
@@ -277,6 +291,9 @@ __device__ void level_bu(const Graph &g, const Buffers &b, uint32_t d, Emitter &
          base += static_cast<uint64_t>(gridDim.x) * kBlock) {
         const uint64_t v64 = base + threadIdx.x;
         const uint32_t v = static_cast<uint32_t>(v64);
+        // The depths a dense top-down level left: its discoveries are this
+        // level's frontier.
+        if (fill && v64 < V && test_bit(front, v)) b.levels[v] = static_cast<int32_t>(d);
         const bool unvisited = v64 < V && !test_bit(b.vis, v);
         // The likely parent first: one coalesced load and one probe resolve
         // most vertices of a big level; the in-edges are scanned otherwise.
@@ -367,8 +384,18 @@ __global__ __launch_bounds__(kBlock) void bfs_engine_step(Graph g, Buffers b)
     if (threadIdx.x == 0) lcount = 0;
     __syncthreads();
     Emitter em{lbuf, &lcount, &lbase};
-    if (variant == TD) level_td(g, b, d, em, sstart, sscan, &stotal);
-    else level_bu(g, b, d, em, variant == BU_WAVE);
+    const bool fill = st.fill_levels != 0;
+    if (variant == TD) {
+        if (fill) {  // the previous level's depths, from this level's frontier bitmap
+            const uint32_t *front = b.bits[d % 3];
+            for (uint64_t v = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x; v < g.V;
+                 v += static_cast<uint64_t>(gridDim.x) * kBlock)
+                if (test_bit(front, static_cast<uint32_t>(v))) b.levels[v] = static_cast<int32_t>(d);
+        }
+        level_td(g, b, d, em, sstart, sscan, &stotal);
+    } else {
+        level_bu(g, b, d, em, variant == BU_WAVE, fill);
+    }
     em.flush(b, d, kLdsQueue);  // reserve everything: moves all that is staged
 
     // Workgroup partials of the discovered vertices' out-degrees.
@@ -517,6 +544,7 @@ __global__ __launch_bounds__(kBlock) void bfs_engine_epilogue(Graph g, Buffers b
     const uint32_t d = st.depth;
     const uint32_t nhub = min(st.hubout, kHubCap);
     const uint32_t nnew = nhub + st.qout;
+    const bool deferred = defer_depths(st, b, g.V);  // this level's, before the state moves on
     __syncthreads();
 
     // Hub degree prefix of the next frontier (kHubCap / kBlock hubs per
@@ -579,6 +607,7 @@ __global__ __launch_bounds__(kBlock) void bfs_engine_epilogue(Graph g, Buffers b
             st.visited += nnew;
             st.frontier_edges = a;
             st.frontier_sq = q;
+            st.fill_levels = deferred ? 1u : 0u;
         }
     }
     __syncthreads();
@@ -632,6 +661,7 @@ __global__ __launch_bounds__(kBlock) void bfs_engine_init(Graph g, Buffers b, Po
         st.prev_choice = -1;
         st.prev_time_ns = 0.0f;
         st.t_step = 0;
+        st.fill_levels = 0;
         st.qcount = nq;
         st.hubcount = nh;
         st.choice = p.fixed;
