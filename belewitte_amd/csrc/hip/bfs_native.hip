@@ -42,7 +42,7 @@ namespace {
 constexpr int kBlock = 256;
 constexpr uint32_t kLdsQueue = 2048;   // staged discoveries per workgroup
 constexpr uint32_t kHubCap = 4096;     // hubs per level (more go to the normal queue)
-constexpr uint32_t kParentScan = 256;  // in-edges scanned for a vertex's likely parent
+constexpr uint32_t kParentScan = 32;   // in-edges scanned for a vertex's likely parent
 constexpr uint32_t kNoParent = 0xFFFFFFFFu;
 constexpr int kFeatures = BW_BFS_FEATURES;
 constexpr int kHidden = 32;
