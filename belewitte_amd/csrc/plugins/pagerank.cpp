@@ -5,7 +5,8 @@
 // each a tuple (iteration kernel, consolidate, zeroInitDegrees,
 // computeDegrees), with the reference's 100 iterations and output formats.
 // Two flags extend it: --iterations N and --dangling drop|uniform (drop is
-// the reference's behaviour).
+// the reference's behaviour).  It also registers "vertex-pull-nodiv-fused",
+// the MI355X-native fused pull iteration (FusedPageRank below).
 
 #include <algorithm>
 #include <cstring>
@@ -16,6 +17,7 @@
 #include <vector>
 
 #include "Algorithm.hpp"
+#include "belewitte_graph.h"
 #include "HIPBackend.hpp"
 #include "ImplementationTemplate.hpp"
 #include "Timer.hpp"
@@ -192,6 +194,36 @@ float roundPrecision(float v, uint32_t mask)
     return v;
 }
 
+// The `.output` of a PageRank run (pagerank/pagerank.cpp:134-170): ranks
+// with 7 significant digits under --validate, otherwise the vertex ids in
+// bucketed rank order.
+template<typename Ranks>
+void writeRanks(std::ofstream &outputFile, const Ranks &ranks, size_t n, bool validate)
+{
+    const unsigned maskBits = 16, bucketSize = 1000;
+    auto oldLocale = outputFile.imbue(std::locale::classic());
+    if (validate) {
+        outputFile << std::setprecision(std::numeric_limits<float>::digits10 + 1);
+        for (size_t i = 0; i < n; i++) outputFile << i << "\t" << ranks[i] << "\n";
+    } else {
+        const uint32_t mask = mask_N_bits(maskBits);
+        std::vector<std::pair<float, size_t>> buckets;
+        buckets.reserve(n);
+        for (size_t i = 0; i < n; i++) buckets.emplace_back(roundPrecision(ranks[i], mask), i);
+        std::stable_sort(buckets.begin(), buckets.end());
+        for (size_t i = 0; i < buckets.size(); i += bucketSize) {
+            // The last element stays outside the last sort
+            // (pagerank/pagerank.cpp:155-156).
+            const size_t end = std::min(i + bucketSize, buckets.size() - 1);
+            if (end > i)
+                std::stable_sort(buckets.begin() + static_cast<long>(i), buckets.begin() + static_cast<long>(end),
+                                 [](const auto &a, const auto &b) { return a.second < b.second; });
+        }
+        for (const auto &p : buckets) outputFile << p.second << "\n";
+    }
+    outputFile.imbue(oldLocale);
+}
+
 }  // namespace
 
 template<typename Platform, typename Vertex, typename Edge, bool switching>
@@ -210,8 +242,6 @@ struct PageRank : public ImplementationTemplate<Platform, Vertex, Edge, switchin
 
     int max_iterations = 100;  // pagerank/pagerank.cpp:55
     std::string dangling = "drop";
-    const unsigned maskBits = 16;
-    const unsigned bucketSize = 1000;
 
     Kernel<unsigned *> zeroInitDegrees;
     Kernel<unsigned *> computeDegrees;
@@ -301,27 +331,115 @@ struct PageRank : public ImplementationTemplate<Platform, Vertex, Edge, switchin
             resultTransfer.stop();
         }
 
-        auto oldLocale = outputFile.imbue(std::locale::classic());
-        if (validate) {
-            outputFile << std::setprecision(std::numeric_limits<float>::digits10 + 1);
-            for (size_t i = 0; i < pageranks.size; i++) outputFile << i << "\t" << pageranks[i] << "\n";
+        writeRanks(outputFile, pageranks, pageranks.size, validate);
+    }
+};
+
+// "vertex-pull-nodiv-fused": the MI355X-native fused pull iteration of
+// libbelewitte_hip (bw_pr_fused_run), which has no counterpart in the
+// reference.  It computes vertex-pull-nodiv (pagerank/vertex_pull_nodiv.cu):
+// DROP mode keeps the reference's undivided first iteration, UNIFORM mode
+// redistributes the dangling mass.  Same lifecycle, --iterations and
+// --dangling flags and output formats as the implementations above.  By
+// default the vertices are relabelled by decreasing out-degree on upload
+// (--no-reorder keeps the file's order; the output is in the file's order
+// either way) and the row tiles run in source-sorted windows of 2 tiles
+// (--windows 0/1/2/4) on 3 persistent workgroups per CU (--rows-grid N).
+// Timer "computation": device time of the iterations.
+struct FusedPageRank : public ImplementationBase {
+    int max_iterations = 100;
+    std::string dangling = "drop";
+    bool keepOrder = false;
+    int windows = 2;
+    int rowsGrid = 0;
+    bw_graph *host = nullptr;
+    bw_graph_dev dev{};
+    std::vector<uint32_t> perm;  // file id -> device id, when relabelled
+    std::vector<void *> bufs;
+
+    FusedPageRank()
+    {
+        options.add("iterations", "NUM", max_iterations, "Number of PageRank iterations.");
+        options.add("dangling", "MODE", dangling,
+                    "Dangling mass: drop (the reference's behaviour) or uniform (redistributed).");
+        options.add("no-reorder", keepOrder, true, "Keep the file's vertex order on the device.");
+        options.add("windows", "NUM", windows, "Row tiles per source-sorted window (0: off, 1, 2, 4).");
+        options.add("rows-grid", "NUM", rowsGrid, "Persistent row workgroups (0: 3 per CU).");
+    }
+
+    ~FusedPageRank() override { FusedPageRank::freeGraph(); }
+
+    void *upload(const void *src, size_t bytes)
+    {
+        void *d = nullptr;
+        BW_CHK(bw_hip_dev_alloc(&d, std::max<size_t>(bytes, 4)));
+        bufs.push_back(d);
+        if (bytes && src) BW_CHK(bw_hip_memcpy_h2d(d, src, bytes));
+        return d;
+    }
+
+    void loadGraph(const std::string file) override
+    {
+        freeGraph();
+        if (bw_graph_load(&host, file.c_str()) != 0) reportError("Failed to load '", file, "': ", bw_graph_last_error());
+        const uint64_t V = bw_graph_vertex_count(host), E = bw_graph_edge_count(host);
+        const bw_graph *g = host;
+        bw_graph *relabelled = nullptr;
+        if (!keepOrder) {
+            perm.assign(V, 0);
+            if (bw_graph_reorder_by_degree(&relabelled, host, perm.data()) != 0) reportError(bw_graph_last_error());
+            g = relabelled;
         } else {
-            const uint32_t mask = mask_N_bits(maskBits);
-            std::vector<std::pair<float, size_t>> buckets;
-            buckets.reserve(pageranks.size);
-            for (size_t i = 0; i < pageranks.size; i++) buckets.emplace_back(roundPrecision(pageranks[i], mask), i);
-            std::stable_sort(buckets.begin(), buckets.end());
-            for (size_t i = 0; i < buckets.size(); i += bucketSize) {
-                // The last element stays outside the last sort
-                // (pagerank/pagerank.cpp:155-156).
-                const size_t end = std::min(i + bucketSize, buckets.size() - 1);
-                if (end > i)
-                    std::stable_sort(buckets.begin() + static_cast<long>(i), buckets.begin() + static_cast<long>(end),
-                                     [](const auto &a, const auto &b) { return a.second < b.second; });
-            }
-            for (const auto &p : buckets) outputFile << p.second << "\n";
+            perm.clear();
         }
-        outputFile.imbue(oldLocale);
+        std::vector<uint32_t> off(V + 1);
+        if (bw_graph_offsets32(g, 1, off.data()) != 0) reportError(bw_graph_last_error());
+        dev.vertex_count = V;
+        dev.edge_count = E;
+        dev.offset_bytes = 4;
+        dev.rev_offsets = upload(off.data(), 4 * (V + 1));
+        dev.rev_cols = static_cast<uint32_t *>(upload(bw_graph_cols(g, 1), 4 * E));
+        if (relabelled) bw_graph_free(relabelled);
+    }
+
+    void freeGraph() override
+    {
+        for (void *p : bufs) (void)bw_hip_dev_free(p);
+        bufs.clear();
+        if (host) bw_graph_free(host);
+        host = nullptr;
+        dev = bw_graph_dev{};
+    }
+
+    void runImplementation(std::ofstream &outputFile) override
+    {
+        if (dangling != "drop" && dangling != "uniform") reportError("--dangling must be drop or uniform");
+        checkError(max_iterations >= 1, "--iterations must be at least 1");
+        const int32_t mode = dangling == "uniform" ? BW_PR_DANGLING_UNIFORM : BW_PR_DANGLING_DROP;
+        const uint64_t V = dev.vertex_count;
+        bw_pr_plan *plan = nullptr;
+        BW_CHK(bw_pr_plan_create(&plan, V, dev.edge_count, dev.rev_offsets, 4, dev.rev_cols));
+        int32_t device = 0;
+        bw_hip_props props;
+        BW_CHK(bw_hip_get_device_props(device, &props));
+        const uint32_t grid = rowsGrid > 0 ? static_cast<uint32_t>(rowsGrid) : 3u * static_cast<uint32_t>(props.compute_units);
+        BW_CHK(bw_pr_plan_tune(plan, 0, grid, 1));
+        BW_CHK(bw_pr_plan_sort_windows(plan, windows, 16));
+        float *ranks = static_cast<float *>(upload(nullptr, 4 * V));
+        float *scratch = static_cast<float *>(upload(nullptr, 4 * V));
+        double *acc = static_cast<double *>(upload(nullptr, 8 * bw_pr_fused_acc_len(max_iterations)));
+        Timer computation("computation", run_count);
+        for (size_t i = 0; i < run_count; i++) {
+            float ms = 0;
+            BW_CHK(bw_pr_fused_run(&dev, plan, max_iterations, mode, mode == BW_PR_DANGLING_DROP ? 1 : 0, ranks,
+                                   scratch, acc, nullptr, &ms, nullptr));
+            computation.add(static_cast<double>(ms) * 1e6);
+        }
+        BW_CHK(bw_pr_plan_destroy(plan));
+        std::vector<float> dranks(V), out(V);
+        BW_CHK(bw_hip_memcpy_d2h(dranks.data(), ranks, 4 * V));
+        for (uint64_t v = 0; v < V; v++) out[v] = perm.empty() ? dranks[v] : dranks[perm[v]];
+        writeRanks(outputFile, out, V, validate);
     }
 };
 
@@ -376,4 +494,5 @@ extern "C" void registerHIP(Algorithm &result)
 
     for (auto &[name, kernel] : prMap) result.addImplementation(name, make_implementation<PageRank>(kernel));
     result.addImplementation("switch", make_switch_implementation<PageRank>(prMap));
+    result.addImplementation("vertex-pull-nodiv-fused", std::make_unique<FusedPageRank>());
 }

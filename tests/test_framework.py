@@ -41,7 +41,7 @@ def test_bfs_implementation_names():
 def test_pagerank_implementation_names():
     r = run("list", "implementations", "-a", "pagerank")
     assert r.returncode == 0
-    assert r.stdout.split() == sorted(list(pagerank.VARIANTS) + ["switch"])
+    assert r.stdout.split() == sorted(list(pagerank.VARIANTS) + ["switch", "vertex-pull-nodiv-fused"])
 
 
 def test_implementation_flags_in_help():

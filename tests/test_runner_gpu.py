@@ -168,3 +168,29 @@ def test_native_engine_through_the_runner(gfile, tmp_path):
         timings = (out / "rmat11.timings").read_text().splitlines()
         assert all(TIMING.match(t) for t in timings), timings
         assert sum(1 for t in timings if ":bfsLevel:" in t) == nlev
+
+
+def test_fused_pagerank_through_the_runner(gfile, tmp_path):
+    """vertex-pull-nodiv-fused computes vertex-pull-nodiv: DROP mode with the
+    reference's undivided first iteration; its compensated sums are never
+    less accurate than the reference's sequential FP32 ones, so the bar is
+    1e-6 plus that path's own error against the FP64 recurrence."""
+    g, p = gfile
+    csr = O.CSR.from_graph(g)
+    vid = pagerank.variant_id("vertex-pull-nodiv")
+    cases = [([], 100, "drop"), (["--dangling", "uniform", "--iterations", "30"], 30, "uniform"),
+             (["--no-reorder", "--windows", "0"], 100, "drop"), (["--windows", "4", "--rows-grid", "7"], 100, "drop")]
+    for extra, iters, mode in cases:
+        run(["-a", "pagerank", "-k", "vertex-pull-nodiv-fused", "--validate", *extra, "-o", str(tmp_path), str(p)])
+        vals = np.array([float(l.split("\t")[1]) for l in (tmp_path / "rmat11.output").read_text().splitlines()])
+        want, _ = O.pagerank_f32(csr, vid, iters, mode)
+        x = O.pagerank_f64(csr, iters, mode, nodiv_quirk=(mode == "drop"))
+        ref_err = float(np.max(np.abs(want.astype(np.float64) - x)))
+        assert vals.size == g.vertex_count
+        assert float(np.max(np.abs(vals - want))) <= 1e-6 + ref_err, extra
+    # Default output: the bucketed order of the same ranks.
+    run(["-a", "pagerank", "-k", "vertex-pull-nodiv-fused", "-o", str(tmp_path), str(p)])
+    order = [int(l) for l in (tmp_path / "rmat11.output").read_text().split()]
+    assert sorted(order) == list(range(g.vertex_count))
+    timings = (tmp_path / "rmat11.timings").read_text().splitlines() if (tmp_path / "rmat11.timings").exists() else []
+    assert all(TIMING.match(t) for t in timings)
