@@ -1,0 +1,249 @@
+// Host-side plan of the binned PageRank iteration (propagation blocking:
+// S. Beamer, K. Asanovic, D. Patterson, "Reducing PageRank Communication via
+// Propagation Blocking", IPDPS 2017), restated for LDS accumulation on
+// gfx950.  Plain C++, no HIP types, so that a CPU program can build and
+// check a plan.
+//
+// The vertices (rows) are cut into partitions of 2^rows_log2 rows.  A pair
+// is a distinct (source s, partition k) such that s has at least one
+// out-edge into k; each pair holds one value per iteration, s's
+// contribution rank[s]/outdeg(s), in a "bin" array ordered by partition,
+// then by source.  One iteration is:
+//   1. accumulate: per partition, its in-edges ordered by (source, row)
+//      stream their 16-bit local row ids; each edge adds the value of its
+//      pair to the row's LDS accumulator.  Long partitions are cut into
+//      chunks, one workgroup each, which write one partial vector per chunk.
+//   2. finalise + scatter: per block of consecutive vertices (inside one
+//      partition), the partials of the block's rows are summed in chunk
+//      order and finalised into new ranks; the new contributions of the
+//      block's vertices are staged in LDS and written to every bin slot of
+//      the block's pairs -- a set of runs, one per partition the block's
+//      vertices point into, each contiguous in the bin array.
+// Pairs deduplicate the edges a source has into one partition, so the bin
+// traffic is 4 bytes per pair written and read instead of one gathered
+// cache line per few edges.
+#pragma once
+
+#include <algorithm>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace bw {
+namespace bins {
+
+constexpr uint32_t kTaskEdges = 2048;    // edges of one wave task
+constexpr uint32_t kMaxRuns = 2048;      // partitions one scatter block may point into
+constexpr uint32_t kBlockVerts = 4096;   // vertices of one scatter block at most
+constexpr uint32_t kBlockWork = 65536;   // vertices x chunks of their partition + pairs, per block
+constexpr uint16_t kNewPair = 0x8000;    // edge code flag: the edge opens a new pair
+
+struct Quad {
+    uint32_t x, y, z, w;
+};
+
+struct Plan {
+    uint32_t V = 0, E = 0, rows_log2 = 0, K = 0;
+    uint32_t pairs = 0;                  // U, the number of (source, partition) pairs
+    uint32_t slots = 0;                  // partial vectors (chunks)
+    std::vector<uint16_t> codes;         // per edge, partition-major, each partition padded to 8:
+                                         // (row - partition base) | kNewPair on a pair's first edge
+    std::vector<uint32_t> edge_start;    // K + 1: padded start of each partition's edges
+    std::vector<uint32_t> pair_start;    // K + 1: first pair of each partition
+    std::vector<Quad> tasks;             // (edge begin, edge end, pair base, partition):
+                                         // pair(e) = base + #flags in [begin, e]
+    std::vector<Quad> chunks;            // (task begin, task end, partition, slot), longest first
+    std::vector<uint32_t> chunk_base;    // K + 1: slots of partition k are [chunk_base[k], chunk_base[k+1])
+    std::vector<Quad> blocks;            // (first vertex, end vertex, run begin, run count)
+    std::vector<uint32_t> run_flat;      // per block: flat start of each run, then the block's pair total
+    std::vector<uint32_t> run_bin;       // per block: bin index of each run's first pair, then 0
+    std::vector<uint16_t> psrc;          // per pair (bin order): source - first vertex of its block
+};
+
+// Builds the plan from a reverse CSR (row r's in-edge sources are
+// cols[off[r] .. off[r+1])).  Returns false with `err` set on bad input or
+// on a graph the binned layout cannot hold.
+inline bool build_plan(const uint32_t *off, const uint32_t *cols, uint32_t V, uint32_t E, uint32_t rows_log2,
+                       uint32_t chunk_tasks, Plan &P, std::string &err)
+{
+    if (rows_log2 < 8 || rows_log2 > 15) {
+        err = "rows_log2 must be in [8, 15]";
+        return false;
+    }
+    if (chunk_tasks == 0) {
+        err = "chunk_tasks must be positive";
+        return false;
+    }
+    if (V == 0 || off[V] != E) {
+        err = "reverse offsets do not end at the edge count";
+        return false;
+    }
+    const uint32_t L = rows_log2, D = 1u << L;
+    const uint32_t K = static_cast<uint32_t>((static_cast<uint64_t>(V) + D - 1) >> L);
+    if (K > kMaxRuns) {
+        err = "too many partitions for the binned layout: raise rows_log2";
+        return false;
+    }
+    P = Plan();
+    P.V = V;
+    P.E = E;
+    P.rows_log2 = L;
+    P.K = K;
+
+    // Forward lists: the rows of each source's out-edges, in increasing
+    // row order (so their partitions come in increasing order too).
+    std::vector<uint32_t> foff(static_cast<size_t>(V) + 1, 0);
+    for (uint32_t e = 0; e < E; e++) {
+        if (cols[e] >= V) {
+            err = "column id out of range";
+            return false;
+        }
+        foff[cols[e] + 1]++;
+    }
+    for (uint32_t v = 0; v < V; v++) foff[v + 1] += foff[v];
+    std::vector<uint32_t> frow(std::max<uint32_t>(E, 1));
+    {
+        std::vector<uint32_t> cur(foff.begin(), foff.end() - 1);
+        for (uint32_t r = 0; r < V; r++)
+            for (uint32_t e = off[r]; e < off[r + 1]; e++) frow[cur[cols[e]]++] = r;
+    }
+
+    // Pairs per partition and per source.
+    std::vector<uint64_t> pairsK(K, 0);
+    std::vector<uint32_t> pairsS(V, 0);
+    for (uint32_t s = 0; s < V; s++) {
+        uint32_t prev = UINT32_MAX;
+        for (uint32_t e = foff[s]; e < foff[s + 1]; e++) {
+            const uint32_t k = frow[e] >> L;
+            if (k != prev) {
+                pairsK[k]++;
+                pairsS[s]++;
+                prev = k;
+            }
+        }
+    }
+    P.pair_start.assign(K + 1, 0);
+    P.edge_start.assign(K + 1, 0);
+    uint64_t u = 0, ep = 0;
+    std::vector<uint32_t> edgesK(K);
+    for (uint32_t k = 0; k < K; k++) {
+        P.pair_start[k] = static_cast<uint32_t>(u);
+        P.edge_start[k] = static_cast<uint32_t>(ep);
+        const uint32_t r0 = k << L, r1 = std::min<uint64_t>(static_cast<uint64_t>(r0) + D, V);
+        edgesK[k] = off[r1] - off[r0];
+        u += pairsK[k];
+        ep += (static_cast<uint64_t>(edgesK[k]) + 7) & ~7ull;
+        if (u >= 0x7FFFFFFFull || ep >= 0xFFFFFFF0ull) {
+            err = "graph too large for the binned layout";
+            return false;
+        }
+    }
+    P.pair_start[K] = static_cast<uint32_t>(u);
+    P.edge_start[K] = static_cast<uint32_t>(ep);
+    P.pairs = static_cast<uint32_t>(u);
+
+    // Chunks per partition: ceil(tasks / chunk_tasks).
+    P.chunk_base.assign(K + 1, 0);
+    for (uint32_t k = 0; k < K; k++) {
+        const uint32_t ntasks = (edgesK[k] + kTaskEdges - 1) / kTaskEdges;
+        P.chunk_base[k + 1] = P.chunk_base[k] + (ntasks + chunk_tasks - 1) / chunk_tasks;
+    }
+    P.slots = P.chunk_base[K];
+
+    // Scatter blocks: consecutive vertices of one partition, cut so that
+    // vertices x chunks of the partition + pairs stays within kBlockWork,
+    // at most kBlockVerts vertices and kMaxRuns runs.
+    std::vector<uint32_t> stamp(K, UINT32_MAX), blen(K, 0), done(K, 0), touched;
+    std::vector<uint32_t> blockOf(V);
+    for (uint32_t k = 0; k < K; k++) {
+        const uint32_t vEnd = std::min<uint64_t>((static_cast<uint64_t>(k) << L) + D, V);
+        const uint64_t perVertex = std::max<uint32_t>(1, P.chunk_base[k + 1] - P.chunk_base[k]);
+        uint32_t s = k << L;
+        while (s < vEnd) {
+            const uint32_t id = static_cast<uint32_t>(P.blocks.size()), v0 = s;
+            uint64_t work = 0;
+            touched.clear();
+            while (s < vEnd && s - v0 < kBlockVerts) {
+                uint32_t fresh = 0, prev = UINT32_MAX;
+                for (uint32_t e = foff[s]; e < foff[s + 1]; e++) {
+                    const uint32_t kk = frow[e] >> L;
+                    if (kk != prev && stamp[kk] != id) fresh++;
+                    prev = kk;
+                }
+                const uint64_t w = perVertex + pairsS[s];
+                if (s > v0 && (work + w > kBlockWork || touched.size() + fresh > kMaxRuns)) break;
+                prev = UINT32_MAX;
+                for (uint32_t e = foff[s]; e < foff[s + 1]; e++) {
+                    const uint32_t kk = frow[e] >> L;
+                    if (kk == prev) continue;
+                    prev = kk;
+                    if (stamp[kk] != id) {
+                        stamp[kk] = id;
+                        blen[kk] = 0;
+                        touched.push_back(kk);
+                    }
+                    blen[kk]++;
+                }
+                blockOf[s] = id;
+                work += w;
+                s++;
+            }
+            std::sort(touched.begin(), touched.end());
+            const uint32_t runBegin = static_cast<uint32_t>(P.run_flat.size());
+            uint32_t flat = 0;
+            for (uint32_t kk : touched) {
+                P.run_flat.push_back(flat);
+                P.run_bin.push_back(P.pair_start[kk] + done[kk]);
+                flat += blen[kk];
+                done[kk] += blen[kk];
+            }
+            P.run_flat.push_back(flat);
+            P.run_bin.push_back(0);
+            P.blocks.push_back({v0, s, runBegin, static_cast<uint32_t>(touched.size())});
+        }
+    }
+
+    // Edge codes and pair sources, in (partition, source, row) order.
+    P.codes.assign(std::max<uint64_t>(ep, 8), 0);
+    P.psrc.assign(std::max<uint32_t>(P.pairs, 1), 0);
+    {
+        std::vector<uint32_t> ec(K, 0), pc(K, 0);
+        for (uint32_t s = 0; s < V; s++) {
+            const uint32_t v0 = P.blocks[blockOf[s]].x;
+            uint32_t prev = UINT32_MAX;
+            for (uint32_t e = foff[s]; e < foff[s + 1]; e++) {
+                const uint32_t r = frow[e], k = r >> L;
+                const bool fresh = k != prev;
+                prev = k;
+                if (fresh) P.psrc[P.pair_start[k] + pc[k]++] = static_cast<uint16_t>(s - v0);
+                P.codes[P.edge_start[k] + ec[k]++] = static_cast<uint16_t>((r & (D - 1)) | (fresh ? kNewPair : 0));
+            }
+        }
+    }
+
+    // Wave tasks of kTaskEdges edges, and the chunks over them.
+    for (uint32_t k = 0; k < K; k++) {
+        const uint32_t t0 = static_cast<uint32_t>(P.tasks.size());
+        uint32_t flags = 0;
+        for (uint32_t b = 0; b < edgesK[k]; b += kTaskEdges) {
+            const uint32_t n = std::min(kTaskEdges, edgesK[k] - b);
+            const uint32_t eb = P.edge_start[k] + b;
+            P.tasks.push_back({eb, eb + n, P.pair_start[k] - 1u + flags, k});
+            for (uint32_t i = 0; i < n; i++) flags += (P.codes[eb + i] & kNewPair) ? 1u : 0u;
+        }
+        const uint32_t t1 = static_cast<uint32_t>(P.tasks.size());
+        uint32_t slot = P.chunk_base[k];
+        for (uint32_t t = t0; t < t1; t += chunk_tasks)
+            P.chunks.push_back({t, std::min(t1, t + chunk_tasks), k, slot++});
+        if (slot != P.chunk_base[k + 1]) {
+            err = "internal error: chunk count";
+            return false;
+        }
+    }
+    std::stable_sort(P.chunks.begin(), P.chunks.end(),
+                     [](const Quad &a, const Quad &b) { return a.y - a.x > b.y - b.x; });
+    return true;
+}
+
+}  // namespace bins
+}  // namespace bw

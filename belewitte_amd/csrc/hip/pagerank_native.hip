@@ -29,6 +29,7 @@
 // combined in a fixed order, so ranks are bit-reproducible run to run.
 
 #include "common.hpp"
+#include "pagerank_bins_plan.hpp"
 
 #include <algorithm>
 #include <thread>
@@ -100,19 +101,24 @@ struct IterArgs {
     int cin_divided;
 };
 
+// The new rank from the gathered sum: DROP follows consolidateRankNoDiv,
+// pagerank/pagerank.cu:154 (float base, fma); UNIFORM adds the dangling mass
+// D spread over all vertices, in FP64.
+__device__ __forceinline__ float new_rank(int mode, uint32_t V, float sum, double D)
+{
+    if (mode == BW_PR_DANGLING_DROP) {
+        const float baseF = (1.0f - kDamp) / static_cast<float>(V);
+        return __builtin_fmaf(kDamp, sum, baseF);
+    }
+    const double d = static_cast<double>(kDamp);
+    return static_cast<float>((1.0 - d) / static_cast<double>(V) +
+                              d * (static_cast<double>(sum) + D / static_cast<double>(V)));
+}
+
 __device__ __forceinline__ void finalise(const IterArgs &a, uint32_t v, float sum, uint32_t dv, float old,
                                          double D, double &resid, double &dangl)
 {
-    const double d = static_cast<double>(kDamp);
-    float nr;
-    if (a.mode == BW_PR_DANGLING_DROP) {
-        // consolidateRankNoDiv, pagerank/pagerank.cu:154 (float base, fma).
-        const float baseF = (1.0f - kDamp) / static_cast<float>(a.V);
-        nr = __builtin_fmaf(kDamp, sum, baseF);
-    } else {
-        nr = static_cast<float>((1.0 - d) / static_cast<double>(a.V) +
-                                d * (static_cast<double>(sum) + D / static_cast<double>(a.V)));
-    }
+    const float nr = new_rank(a.mode, a.V, sum, D);
     double oldRank = static_cast<double>(old);
     if (a.cin_divided && dv != 0) oldRank *= static_cast<double>(dv);
     resid += fabs(static_cast<double>(nr) - oldRank);
@@ -164,12 +170,13 @@ __device__ __forceinline__ void block_kahan(float &s, float &c, float (*wsum)[2]
 }
 
 // Sum of n (residual, dangling) partials in a fixed order: thread t adds
-// partials t, t + kBlock, ...; then a fixed tree.  Results in thread 0.
+// partials t, t + kThreads, ...; then a fixed tree.  Results in thread 0.
+template<int kThreads = kBlock>
 __device__ __forceinline__ void reduce_partials(const double2 *p, uint32_t n, double &r, double &g, double (*red)[2])
 {
     r = 0.0;
     g = 0.0;
-    for (uint32_t i = threadIdx.x; i < n; i += kBlock) {
+    for (uint32_t i = threadIdx.x; i < n; i += kThreads) {
         const double2 t = p[i];
         r += t.x;
         g += t.y;
@@ -185,7 +192,7 @@ __device__ __forceinline__ void reduce_partials(const double2 *p, uint32_t n, do
     if (threadIdx.x == 0) {
         r = 0.0;
         g = 0.0;
-        for (int w = 0; w < kBlock / kWave; w++) {
+        for (int w = 0; w < kThreads / kWave; w++) {
             r += red[w][0];
             g += red[w][1];
         }
@@ -317,7 +324,7 @@ __device__ __forceinline__ void finalise_long(const IterArgs &a, uint32_t item, 
 
 // The workgroup's (residual, dangling) partial, fixed reduction order.
 template<int kThreads>
-__device__ __forceinline__ void write_partial(const IterArgs &a, double resid, double dangl, double (*red)[2])
+__device__ __forceinline__ void write_partial(double2 *part, double resid, double dangl, double (*red)[2])
 {
     resid = wave_sum_f64(resid);
     dangl = wave_sum_f64(dangl);
@@ -333,7 +340,7 @@ __device__ __forceinline__ void write_partial(const IterArgs &a, double resid, d
             r += red[w][0];
             q += red[w][1];
         }
-        a.part[blockIdx.x] = make_double2(r, q);
+        part[blockIdx.x] = make_double2(r, q);
     }
 }
 
@@ -387,7 +394,7 @@ __global__ __launch_bounds__(kBlock * kSub) void pr_rows_kernel(IterArgs a)
         if (isTile) reduce_rows(a, M, vals[g], r0, nrows, tid, D, resid, dangl);
         __syncthreads();  // the next step reuses the LDS staging
     }
-    write_partial<kBlock * kSub>(a, resid, dangl, red);
+    write_partial<kBlock * kSub>(a.part, resid, dangl, red);
 }
 
 // Row tiles grouped into windows of up to kWinTiles consecutive tiles with
@@ -455,7 +462,271 @@ __global__ __launch_bounds__(kBlock * kWinTiles) void pr_windows_kernel(IterArgs
         if (isTile) reduce_rows(a, M, vals + voff, r0, nrows, tid, D, resid, dangl);
         __syncthreads();
     }
-    write_partial<kBlock * kWinTiles>(a, resid, dangl, red);
+    write_partial<kBlock * kWinTiles>(a.part, resid, dangl, red);
+}
+
+// ------------------------------------------------ binned iteration
+// Propagation blocking (the plan is described in pagerank_bins_plan.hpp):
+// per iteration an accumulate launch (chunks of one partition's edges into
+// an LDS row vector, written out as one partial per chunk) and a finalise
+// launch (partials summed in chunk order into new ranks, whose
+// contributions are scattered to the bins of the next iteration).  The LDS
+// row accumulators are FP64 (ds_add_f64): their order of addition within a
+// chunk changes from run to run, but each partial is rounded to FP32 from
+// an FP64 sum, so the ranks practically never do.  (FP32 LDS atomics were
+// measured 3.4x slower than FP64 ones in this kernel on gfx950, besides
+// drifting on hub rows.)
+
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+struct BinArgs {
+    uint32_t V;
+    const uint16_t *codes;      // per edge: local row | new-pair flag
+    const uint4 *tasks;         // (edge begin, edge end, pair base, partition)
+    const uint4 *chunks;        // (task begin, task end, partition, slot)
+    uint32_t nchunks;
+    float *bins;                // one contribution per pair
+    float *partials;            // [slot][rows per partition]
+    const uint32_t *chunk_base; // per partition: its first slot (K + 1)
+    const uint4 *blocks;        // (first vertex, end vertex, run begin, run count)
+    uint32_t nblocks;
+    const uint32_t *run_flat;
+    const uint32_t *run_bin;
+    const uint16_t *psrc;
+    const uint32_t *deg;
+    float *rank;                // ranks, updated in place
+    const double2 *prev;        // per-workgroup (residual, dangling) of the previous finalise
+    uint32_t nprev;
+    double *resid_out;
+    double *dangl_out;
+    const double *dangl;        // dangling mass of the ranks the bins hold
+    double2 *part;
+    int mode;
+    int phase;    // 0: initial ranks 1/V; 1: finalise from the partials
+    int scatter;  // write the bins for the next iteration
+    int divide;   // phase 0: contributions divided by the out-degree
+};
+
+// Workgroup per chunk; the last workgroup reduces the previous finalise's
+// (residual, dangling) partials.  Each wave takes whole tasks; per round a
+// lane holds kLaneEdges consecutive edge codes (16-byte loads, the next
+// round's already in flight), finds the pair of each edge from the
+// new-pair flags (wave prefix over per-lane flag counts, by ballots of
+// their bits), reads the pairs' values and adds each to its row.
+template<int kRowsLog, int kThreads, int kLaneEdges>
+__global__ __launch_bounds__(kThreads, kRowsLog == 13 ? 8 : 4) void pr_bins_accumulate_kernel(BinArgs a)
+{
+    constexpr uint32_t kRows = 1u << kRowsLog;
+    constexpr int kWaves = kThreads / kWave;
+    constexpr int kVec = kLaneEdges / 8;                 // 16-byte loads per lane and round
+    constexpr uint32_t kRound = kWave * kLaneEdges;      // edges per wave and round
+    constexpr int kCountBits = kLaneEdges == 8 ? 4 : 5;  // flag count 0..kLaneEdges
+    static_assert(bins::kTaskEdges % kRound == 0, "a task is whole rounds");
+    __shared__ double acc[kRows];
+    __shared__ double red[kWaves][2];
+    if (blockIdx.x == gridDim.x - 1) {
+        double r, g;
+        reduce_partials<kThreads>(a.prev, a.nprev, r, g, red);
+        if (threadIdx.x == 0) {
+            *a.resid_out = r;
+            *a.dangl_out = g;
+        }
+        return;
+    }
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+    const uint64_t below = (1ull << lane) - 1ull;
+    // The lane's codes of the round starting at edge e0 (zeros past end;
+    // every 8-edge group starts 8-aligned inside the padded partition).
+    auto load_codes = [&](uint32_t e0, uint32_t end, u32x4 (&raw)[kVec]) {
+#pragma unroll
+        for (int q = 0; q < kVec; q++) {
+            const uint32_t e = e0 + lane * kLaneEdges + 8 * q;
+            raw[q] = u32x4{0u, 0u, 0u, 0u};
+            if (e < end) raw[q] = __builtin_nontemporal_load(reinterpret_cast<const u32x4 *>(a.codes + e));
+        }
+    };
+    for (uint32_t c = blockIdx.x; c < a.nchunks; c += gridDim.x - 1) {
+        const uint4 ch = a.chunks[c];
+        for (uint32_t i = threadIdx.x; i < kRows; i += kThreads) acc[i] = 0.0;
+        __syncthreads();
+        // Rounds are software-pipelined: the codes of round r + 2 and the
+        // pair values of round r + 1 are in flight while round r adds.
+        for (uint32_t t = ch.x + wid; t < ch.y; t += kWaves) {
+            const uint4 tk = a.tasks[t];
+            uint32_t base = tk.z;
+            u32x4 raw[kVec];
+            uint32_t code[kLaneEdges], n;
+            float v[kLaneEdges];
+            // Decodes the round at e0 from raw, computes its pairs and
+            // issues its value loads into (code, v, n).
+            auto stage = [&](uint32_t e0, uint32_t (&cd)[kLaneEdges], float (&vv)[kLaneEdges], uint32_t &nn) {
+                const uint32_t e = e0 + lane * kLaneEdges;
+                nn = e < tk.y ? min(static_cast<uint32_t>(kLaneEdges), tk.y - e) : 0u;
+#pragma unroll
+                for (int q = 0; q < kVec; q++)
+#pragma unroll
+                    for (int h = 0; h < 4; h++) {
+                        cd[8 * q + 2 * h] = raw[q][h] & 0xFFFFu;
+                        cd[8 * q + 2 * h + 1] = raw[q][h] >> 16;
+                    }
+                uint32_t cnt = 0;
+#pragma unroll
+                for (int j = 0; j < kLaneEdges; j++)
+                    cnt += (j < static_cast<int>(nn) && (cd[j] & bins::kNewPair)) ? 1u : 0u;
+                uint32_t pre = 0, tot = 0;
+#pragma unroll
+                for (int b = 0; b < kCountBits; b++) {
+                    const uint64_t m = __ballot((cnt >> b) & 1u);
+                    pre += static_cast<uint32_t>(__popcll(m & below)) << b;
+                    tot += static_cast<uint32_t>(__popcll(m)) << b;
+                }
+                uint32_t p = base + pre;
+#pragma unroll
+                for (int j = 0; j < kLaneEdges; j++) {
+                    p += (cd[j] & bins::kNewPair) ? 1u : 0u;
+                    vv[j] = j < static_cast<int>(nn) ? a.bins[p] : 0.0f;
+                }
+                base += tot;
+            };
+            load_codes(tk.x, tk.y, raw);
+            stage(tk.x, code, v, n);
+            load_codes(tk.x + kRound, tk.y, raw);
+            for (uint32_t e0 = tk.x; e0 < tk.y; e0 += kRound) {
+                uint32_t code2[kLaneEdges], n2 = 0;
+                float v2[kLaneEdges];
+                const bool more = e0 + kRound < tk.y;  // wave-uniform
+                if (more) {
+                    stage(e0 + kRound, code2, v2, n2);
+                    load_codes(e0 + 2 * kRound, tk.y, raw);
+                }
+#pragma unroll
+                for (int j = 0; j < kLaneEdges; j++)
+                    if (j < static_cast<int>(n)) atomicAdd(&acc[code[j] & (kRows - 1)], static_cast<double>(v[j]));
+                if (more) {
+#pragma unroll
+                    for (int j = 0; j < kLaneEdges; j++) {
+                        code[j] = code2[j];
+                        v[j] = v2[j];
+                    }
+                    n = n2;
+                }
+            }
+        }
+        __syncthreads();
+        float *out = a.partials + static_cast<size_t>(ch.w) * kRows;
+        for (uint32_t i = threadIdx.x; i < kRows; i += kThreads) out[i] = static_cast<float>(acc[i]);
+        __syncthreads();  // acc is cleared for the next chunk
+    }
+}
+
+// Workgroup per scatter block: new ranks of the block's vertices (phase 1)
+// or the initial ranks (phase 0), their contributions staged in LDS, then
+// written to each of the block's pairs (runs in flat order, each wave a
+// contiguous range of them).  Each vertex's partials are summed
+// by a group of G lanes (G a power of two, as many as the block's vertex
+// count leaves, at most one per partial), lane i adding partials i, i + G,
+// ... in order and the group combining by a fixed xor tree.
+template<int kRowsLog, int kThreads>
+__global__ __launch_bounds__(kThreads) void pr_bins_finalise_kernel(BinArgs a)
+{
+    constexpr int kScatterUnroll = 8;
+    constexpr uint32_t kWaves = kThreads / kWave;
+    const uint32_t lane = threadIdx.x & (kWave - 1);
+    const uint32_t wid = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+    __shared__ float contrib[bins::kBlockVerts];
+    __shared__ uint32_t rflat[bins::kMaxRuns + 1];
+    __shared__ uint32_t rbin[bins::kMaxRuns + 1];
+    __shared__ double red[kThreads / kWave][2];
+    const double D = (a.phase && a.mode == BW_PR_DANGLING_UNIFORM) ? *a.dangl : 0.0;
+    double resid = 0.0, dangl = 0.0;
+    for (uint32_t b = blockIdx.x; b < a.nblocks; b += gridDim.x) {
+        const uint4 bl = a.blocks[b];
+        const uint32_t k = bl.x >> kRowsLog, rbase = k << kRowsLog;
+        const uint32_t c0 = a.chunk_base[k], c1 = a.chunk_base[k + 1];
+        const uint32_t nv = bl.y - bl.x, ns = a.phase ? c1 - c0 : 0u;
+        uint32_t G = 1;
+        while (G < kWave && 2 * G * nv <= kThreads && 2 * G <= ns) G *= 2;
+        const uint32_t sub = threadIdx.x & (G - 1);
+        for (uint32_t v0 = 0; v0 < nv; v0 += kThreads / G) {
+            const uint32_t vi = v0 + threadIdx.x / G;
+            const bool live = vi < nv;
+            double s = 0.0;
+            if (live && a.phase) {
+                // partial of slot c0 + q for vertex bl.x + vi: [c0 + q][bl.x + vi - rbase]
+                const float *pv = a.partials + (static_cast<size_t>(c0) << kRowsLog) + (bl.x - rbase) + vi;
+                uint32_t q = sub;
+                for (; q + 3 * G < ns; q += 4 * G) {
+                    float t[4];
+#pragma unroll
+                    for (int i = 0; i < 4; i++) t[i] = pv[static_cast<size_t>(q + i * G) << kRowsLog];
+#pragma unroll
+                    for (int i = 0; i < 4; i++) s += static_cast<double>(t[i]);
+                }
+                for (; q < ns; q += G) s += static_cast<double>(pv[static_cast<size_t>(q) << kRowsLog]);
+            }
+            for (uint32_t o = G / 2; o > 0; o >>= 1) s += __shfl_xor(s, static_cast<int>(o), kWave);
+            if (live && sub == 0) {
+                const uint32_t v = bl.x + vi;
+                const uint32_t dv = a.deg[v];
+                float nr, cv;
+                if (!a.phase) {
+                    nr = 1.0f / static_cast<float>(a.V);
+                    cv = (a.divide && dv != 0) ? nr / static_cast<float>(dv) : nr;
+                } else {
+                    nr = new_rank(a.mode, a.V, static_cast<float>(s), D);
+                    resid += fabs(static_cast<double>(nr) - static_cast<double>(a.rank[v]));
+                    cv = dv != 0 ? nr / static_cast<float>(dv) : 0.0f;
+                }
+                if (dv == 0) dangl += static_cast<double>(nr);
+                a.rank[v] = nr;
+                contrib[vi] = cv;
+            }
+        }
+        if (a.scatter) {
+            for (uint32_t i = threadIdx.x; i <= bl.w; i += kThreads) {
+                rflat[i] = a.run_flat[bl.z + i];
+                rbin[i] = a.run_bin[bl.z + i];
+            }
+            __syncthreads();
+            // Wave w writes the flat range [fbeg, fend) of the block's pairs;
+            // each lane finds the run of its first flat by binary search,
+            // then follows the runs forward (its flats only increase).
+            const uint32_t total = rflat[bl.w];
+            const uint32_t per = ((total + kWaves - 1) / kWaves + kWave - 1) & ~static_cast<uint32_t>(kWave - 1);
+            const uint32_t fbeg = min(wid * per, total), fend = min(fbeg + per, total);
+            uint32_t lo = 0;
+            if (fbeg + lane < fend) {
+                const uint32_t f = fbeg + lane;
+                uint32_t hi = bl.w;  // rflat[lo] <= f < rflat[hi]
+                while (hi - lo > 1) {
+                    const uint32_t mid = (lo + hi) >> 1;
+                    if (rflat[mid] <= f) lo = mid;
+                    else hi = mid;
+                }
+            }
+            for (uint32_t f0 = fbeg; f0 < fend; f0 += kWave * kScatterUnroll) {
+                uint32_t p[kScatterUnroll];
+                uint16_t src[kScatterUnroll];
+#pragma unroll
+                for (int u = 0; u < kScatterUnroll; u++) {
+                    const uint32_t f = f0 + lane + u * kWave;
+                    p[u] = 0xFFFFFFFFu;
+                    if (f < fend) {
+                        while (rflat[lo + 1] <= f) lo++;
+                        p[u] = rbin[lo] + (f - rflat[lo]);
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < kScatterUnroll; u++) src[u] = p[u] != 0xFFFFFFFFu ? a.psrc[p[u]] : 0;
+#pragma unroll
+                for (int u = 0; u < kScatterUnroll; u++)
+                    if (p[u] != 0xFFFFFFFFu) a.bins[p[u]] = contrib[src[u]];
+            }
+        }
+        __syncthreads();  // contrib and the run table are reused by the next block
+    }
+    write_partial<kThreads>(a.part, resid, dangl, red);
 }
 
 // Reduction of the last launch's partials (after the loop).
@@ -515,12 +786,35 @@ struct bw_pr_plan {
     float2 *partials = nullptr;
     uint32_t *deg = nullptr;
     double2 *part = nullptr;  // max(nlong + ntiles, kInitBlocks) per-workgroup partials
+    const void *roff = nullptr;  // the graph's reverse offsets (for building the bins)
+    // Binned mode (bw_pr_plan_bin): 2^bin_log2 rows per partition, 0 = off.
+    struct Bins {
+        uint32_t log2 = 0, chunk_tasks = 0, nchunks = 0, nblocks = 0, pairs = 0, slots = 0, K = 0;
+        uint32_t lane_edges = 8;  // edge codes per lane and round of the accumulate kernel
+        uint16_t *codes = nullptr;
+        uint4 *tasks = nullptr, *chunks = nullptr, *blocks = nullptr;
+        uint32_t *chunk_base = nullptr, *run_flat = nullptr, *run_bin = nullptr;
+        uint16_t *psrc = nullptr;
+        float *vals = nullptr, *partials = nullptr;
+        double2 *part = nullptr;
+        void release()
+        {
+            for (void *p : {static_cast<void *>(codes), static_cast<void *>(tasks), static_cast<void *>(chunks),
+                            static_cast<void *>(blocks), static_cast<void *>(chunk_base),
+                            static_cast<void *>(run_flat), static_cast<void *>(run_bin), static_cast<void *>(psrc),
+                            static_cast<void *>(vals), static_cast<void *>(partials), static_cast<void *>(part)})
+                if (p) (void)hipFree(p);
+            *this = Bins();
+        }
+    } bins;
+    bool binned = false;
     ~bw_pr_plan()
     {
         for (void *p : {static_cast<void *>(pieces), static_cast<void *>(lrows), static_cast<void *>(tiles),
                         static_cast<void *>(partials), static_cast<void *>(deg), static_cast<void *>(part),
                         static_cast<void *>(windows), static_cast<void *>(scols), static_cast<void *>(sperm)})
             if (p) (void)hipFree(p);
+        bins.release();
     }
 };
 
@@ -568,6 +862,7 @@ extern "C" int bw_pr_plan_create(bw_pr_plan **out, uint64_t V, uint64_t E, const
 
     auto plan = new bw_pr_plan();
     plan->rcols = rev_cols;
+    plan->roff = rev_offsets;
     plan->tiles_host = tiles;
     plan->V = static_cast<uint32_t>(V);
     plan->E = static_cast<uint32_t>(E);
@@ -719,6 +1014,82 @@ extern "C" int bw_pr_plan_sort_windows(bw_pr_plan *plan, int32_t tiles_per_windo
     return BW_HIP_OK;
 }
 
+extern "C" int bw_pr_plan_bin(bw_pr_plan *plan, int32_t rows_log2, int32_t chunk_tasks, int32_t lane_edges)
+{
+    if (!plan) return fail(BW_HIP_ERR_ARG, "plan is NULL");
+    if (rows_log2 == 0) {
+        plan->binned = false;
+        return BW_HIP_OK;
+    }
+    if (rows_log2 != 13 && rows_log2 != 14) return fail(BW_HIP_ERR_ARG, "rows_log2 must be 0, 13 or 14");
+    if (chunk_tasks < 1 || chunk_tasks > 65536) return fail(BW_HIP_ERR_ARG, "chunk_tasks must be in [1, 65536]");
+    if (lane_edges != 8 && lane_edges != 16) return fail(BW_HIP_ERR_ARG, "lane_edges must be 8 or 16");
+    auto &B = plan->bins;
+    if (B.log2 == static_cast<uint32_t>(rows_log2) && B.chunk_tasks == static_cast<uint32_t>(chunk_tasks)) {
+        B.lane_edges = static_cast<uint32_t>(lane_edges);
+        plan->binned = true;
+        return BW_HIP_OK;
+    }
+    B.release();
+    plan->binned = false;
+    const uint32_t V = plan->V, E = plan->E;
+    std::vector<uint32_t> off(static_cast<size_t>(V) + 1), cols(std::max<uint32_t>(E, 1));
+    BW_HIP_CHECK(hipMemcpy(off.data(), plan->roff, sizeof(uint32_t) * (static_cast<size_t>(V) + 1),
+                           hipMemcpyDeviceToHost));
+    if (E) BW_HIP_CHECK(hipMemcpy(cols.data(), plan->rcols, sizeof(uint32_t) * E, hipMemcpyDeviceToHost));
+    bins::Plan P;
+    std::string err;
+    if (!bins::build_plan(off.data(), cols.data(), V, E, static_cast<uint32_t>(rows_log2),
+                          static_cast<uint32_t>(chunk_tasks), P, err))
+        return fail(BW_HIP_ERR_ARG, "binned PageRank plan: " + err);
+    static_assert(sizeof(bins::Quad) == sizeof(uint4), "Quad and uint4 share a layout");
+    bool ok = true;
+    auto put = [&ok](auto *&dst, const auto &vec, size_t min_elems = 1) {
+        using T = typename std::decay_t<decltype(vec)>::value_type;
+        const size_t n = std::max(vec.size(), min_elems);
+        if (!ok || hipMalloc(reinterpret_cast<void **>(&dst), sizeof(T) * n) != hipSuccess) {
+            ok = false;
+            return;
+        }
+        if (!vec.empty() && hipMemcpy(dst, vec.data(), sizeof(T) * vec.size(), hipMemcpyHostToDevice) != hipSuccess)
+            ok = false;
+    };
+    put(B.codes, P.codes);
+    put(B.tasks, P.tasks);
+    put(B.chunks, P.chunks);
+    put(B.blocks, P.blocks);
+    put(B.chunk_base, P.chunk_base);
+    put(B.run_flat, P.run_flat);
+    put(B.run_bin, P.run_bin);
+    put(B.psrc, P.psrc);
+    const size_t rows = size_t{1} << rows_log2;
+    if (!ok || hipMalloc(&B.vals, sizeof(float) * std::max<size_t>(P.pairs, 1)) != hipSuccess ||
+        hipMalloc(&B.partials, sizeof(float) * rows * std::max<size_t>(P.slots, 1)) != hipSuccess ||
+        hipMalloc(&B.part, sizeof(double2) * std::max<size_t>(P.blocks.size(), 1)) != hipSuccess) {
+        B.release();
+        return fail(BW_HIP_ERR_NOMEM, "device allocation of the binned PageRank plan failed");
+    }
+    B.log2 = static_cast<uint32_t>(rows_log2);
+    B.chunk_tasks = static_cast<uint32_t>(chunk_tasks);
+    B.lane_edges = static_cast<uint32_t>(lane_edges);
+    B.nchunks = static_cast<uint32_t>(P.chunks.size());
+    B.nblocks = static_cast<uint32_t>(P.blocks.size());
+    B.pairs = P.pairs;
+    B.slots = P.slots;
+    B.K = P.K;
+    plan->binned = true;
+    return BW_HIP_OK;
+}
+
+extern "C" int bw_pr_plan_bin_info(const bw_pr_plan *plan, uint32_t *pairs, uint32_t *chunks, uint32_t *blocks)
+{
+    if (!plan) return fail(BW_HIP_ERR_ARG, "plan is NULL");
+    if (pairs) *pairs = plan->binned ? plan->bins.pairs : 0;
+    if (chunks) *chunks = plan->binned ? plan->bins.nchunks : 0;
+    if (blocks) *blocks = plan->binned ? plan->bins.nblocks : 0;
+    return BW_HIP_OK;
+}
+
 extern "C" int bw_pr_plan_degrees(const bw_pr_plan *plan, const uint32_t **deg)
 {
     if (!plan || !deg) return fail(BW_HIP_ERR_ARG, "invalid argument");
@@ -731,6 +1102,90 @@ extern "C" int bw_pr_plan_degrees(const bw_pr_plan *plan, const uint32_t **deg)
 // scratch slot at the end.
 extern "C" uint64_t bw_pr_fused_acc_len(int32_t iterations)
 { return 2 * static_cast<uint64_t>(std::max(iterations, 0)) + 3; }
+
+namespace {
+
+constexpr int kBinAccThreads = 1024;
+constexpr int kBinFinThreads = 512;
+
+// The binned iteration loop: initial ranks and bins, then per iteration one
+// accumulate and one finalise launch; ranks are updated in place.
+int run_binned(const bw_pr_plan *plan, int32_t iterations, int32_t mode, int32_t nodiv_quirk, float *ranks,
+               double *dangl, double *resid, double *spare, double *resid_host, float *kernel_ms, hipStream_t s)
+{
+    const auto &B = plan->bins;
+    using Kernel = void (*)(BinArgs);
+    const bool l16 = B.lane_edges == 16;
+    const Kernel accK = B.log2 == 13 ? (l16 ? pr_bins_accumulate_kernel<13, kBinAccThreads, 16>
+                                            : pr_bins_accumulate_kernel<13, kBinAccThreads, 8>)
+                                     : (l16 ? pr_bins_accumulate_kernel<14, kBinAccThreads, 16>
+                                            : pr_bins_accumulate_kernel<14, kBinAccThreads, 8>);
+    const Kernel finK = B.log2 == 13 ? pr_bins_finalise_kernel<13, kBinFinThreads>
+                                     : pr_bins_finalise_kernel<14, kBinFinThreads>;
+    BinArgs a;
+    a.V = plan->V;
+    a.codes = B.codes;
+    a.tasks = B.tasks;
+    a.chunks = B.chunks;
+    a.nchunks = B.nchunks;
+    a.bins = B.vals;
+    a.partials = B.partials;
+    a.chunk_base = B.chunk_base;
+    a.blocks = B.blocks;
+    a.nblocks = B.nblocks;
+    a.run_flat = B.run_flat;
+    a.run_bin = B.run_bin;
+    a.psrc = B.psrc;
+    a.deg = plan->deg;
+    a.rank = ranks;
+    a.prev = B.part;
+    a.nprev = B.nblocks;
+    a.resid_out = spare;
+    a.dangl_out = spare;
+    a.dangl = dangl;
+    a.part = B.part;
+    a.mode = mode;
+    a.phase = 0;
+    a.scatter = 1;
+    a.divide = !(nodiv_quirk && mode == BW_PR_DANGLING_DROP);
+    const uint32_t finGrid = B.nblocks, accGrid = B.nchunks + 1;
+    hipLaunchKernelGGL(finK, dim3(finGrid), dim3(kBinFinThreads), 0, s, a);
+    BW_LAUNCH_CHECK();
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    if (kernel_ms) {
+        BW_HIP_CHECK(hipEventCreate(&ev0));
+        BW_HIP_CHECK(hipEventCreate(&ev1));
+        BW_HIP_CHECK(hipEventRecord(ev0, s));
+    }
+    for (int32_t it = 1; it <= iterations; it++) {
+        a.resid_out = it > 1 ? resid + (it - 2) : spare;
+        a.dangl_out = dangl + (it - 1);
+        a.dangl = dangl + (it - 1);
+        hipLaunchKernelGGL(accK, dim3(accGrid), dim3(kBinAccThreads), 0, s, a);
+        BW_LAUNCH_CHECK();
+        a.phase = 1;
+        a.scatter = it < iterations;
+        hipLaunchKernelGGL(finK, dim3(finGrid), dim3(kBinFinThreads), 0, s, a);
+        BW_LAUNCH_CHECK();
+    }
+    hipLaunchKernelGGL(pr_reduce_kernel, dim3(1), dim3(kBlock), 0, s, B.part, finGrid, resid + (iterations - 1),
+                       dangl + iterations);
+    BW_LAUNCH_CHECK();
+    if (kernel_ms) {
+        BW_HIP_CHECK(hipEventRecord(ev1, s));
+        BW_HIP_CHECK(hipEventSynchronize(ev1));
+        BW_HIP_CHECK(hipEventElapsedTime(kernel_ms, ev0, ev1));
+        (void)hipEventDestroy(ev0);
+        (void)hipEventDestroy(ev1);
+    }
+    if (resid_host) {
+        BW_HIP_CHECK(hipMemcpyAsync(resid_host, resid, sizeof(double) * iterations, hipMemcpyDeviceToHost, s));
+        BW_HIP_CHECK(hipStreamSynchronize(s));
+    }
+    return BW_HIP_OK;
+}
+
+}  // namespace
 
 extern "C" int bw_pr_fused_run(const bw_graph_dev *g, const bw_pr_plan *plan, int32_t iterations, int32_t mode,
                                int32_t nodiv_quirk, float *ranks, float *scratch, double *acc,
@@ -746,6 +1201,8 @@ extern "C" int bw_pr_fused_run(const bw_graph_dev *g, const bw_pr_plan *plan, in
     hipStream_t s = stream_of(cfg);
     const uint32_t V = plan->V;
     double *dangl = acc, *resid = acc + iterations + 1, *spare = acc + 2 * iterations + 2;
+    if (plan->binned) return run_binned(plan, iterations, mode, nodiv_quirk, ranks, dangl, resid, spare, resid_host,
+                                        kernel_ms, s);
 
     // Ping-pong so the last iteration lands in `ranks`.
     float *buf[2];

@@ -123,6 +123,7 @@ class FusedPageRank:
         self._plan = p
         self.max_iterations = int(max_iterations)
         self.tuning = (0, 0, 1, 0, 16)
+        self.binning = None
         V = max(dg.vertex_count, 1)
         self.ranks = DeviceArray(V, np.float32)
         self.scratch = DeviceArray(V, np.float32)
@@ -141,6 +142,22 @@ class FusedPageRank:
         N.check_hip(lib.bw_pr_plan_tune(self._plan, int(hot), int(rows_grid), int(tiles)))
         N.check_hip(lib.bw_pr_plan_sort_windows(self._plan, int(sorted_windows), int(batch)))
         self.tuning = (int(hot), int(rows_grid), int(tiles), int(sorted_windows), int(batch))
+
+    def bin(self, rows_log2: int = 13, chunk_tasks: int = 64, lane_edges: int = 8) -> None:
+        """Switches to the binned iteration (propagation blocking): rows cut
+        into partitions of 2**rows_log2 (13, 14) accumulated in LDS in FP64,
+        one contribution per distinct (source, partition) pair written and
+        read per iteration, partitions' edges cut into chunks of
+        chunk_tasks x 2048 edges, lane_edges (8, 16) edges per lane and
+        round.  rows_log2=0 returns to the pull kernels.  The plan is built
+        on the host on the first call for a partition and chunk size."""
+        N.check_hip(N.hip_lib().bw_pr_plan_bin(self._plan, int(rows_log2), int(chunk_tasks), int(lane_edges)))
+        self.binning = (int(rows_log2), int(chunk_tasks), int(lane_edges)) if rows_log2 else None
+
+    def bin_info(self) -> dict:
+        a, b, c = C.c_uint32(), C.c_uint32(), C.c_uint32()
+        N.check_hip(N.hip_lib().bw_pr_plan_bin_info(self._plan, C.byref(a), C.byref(b), C.byref(c)))
+        return {"pairs": a.value, "chunks": b.value, "blocks": c.value}
 
     def autotune(self, iterations: int = 10, candidates=None, margin: float = 0.02) -> dict:
         """Times each launch shape over `iterations` iterations; keeps the
@@ -166,6 +183,9 @@ class FusedPageRank:
 
     def kernel_names(self) -> str:
         """The kernels one iteration launches under the current shape."""
+        if self.binning:
+            r, _, lanes = self.binning
+            return f"pr_bins_accumulate_kernel<{r}, 1024, {lanes}> + pr_bins_finalise_kernel<{r}, 256>"
         hot, _, tiles, width, batch = self.tuning
         rows = f"pr_windows_kernel<{width}, {batch}>" if width else f"pr_rows_kernel<{hot}, {tiles}>"
         return f"pr_pieces_kernel + {rows}"

@@ -413,6 +413,25 @@ int bw_pr_plan_tune(bw_pr_plan *plan, uint32_t hot_vertices,
  * in flight at once (8 needs fewer registers, so more waves fit a CU). */
 int bw_pr_plan_sort_windows(bw_pr_plan *plan, int32_t tiles_per_window,
                             int32_t gathers_per_round);
+/* rows_log2 13 or 14: switches the plan to the binned iteration
+ * (propagation blocking), built on the host on the first call for a
+ * partition size and chunk size from the graph's reverse CSR given to
+ * bw_pr_plan_create (which must still be live).  Rows are cut into
+ * partitions of 2^rows_log2; each distinct (source, partition) pair holds
+ * one contribution per iteration, written once by the source and read once
+ * by the partition, whose rows accumulate in LDS in FP64; a partition's
+ * edges are cut into chunks of chunk_tasks x 2048 edges, one workgroup and
+ * one partial row vector each; lane_edges (8 or 16) edge codes per lane and
+ * round.  Same recurrence and modes as the pull kernels
+ * (pagerank/vertex_pull_nodiv.cu:3-29, pagerank/pagerank.cu:142-165); the
+ * LDS additions run in a run-dependent order, but each partial is an FP64
+ * sum rounded once to FP32.  Needs V / 2^rows_log2 <= 2048.  rows_log2 0
+ * returns to the pull kernels. */
+int bw_pr_plan_bin(bw_pr_plan *plan, int32_t rows_log2, int32_t chunk_tasks,
+                   int32_t lane_edges);
+/* Pairs, chunks and scatter blocks of the binned plan (0 when off). */
+int bw_pr_plan_bin_info(const bw_pr_plan *plan, uint32_t *pairs,
+                        uint32_t *chunks, uint32_t *blocks);
 
 /* Number of doubles of the `acc` workspace of bw_pr_fused_run. */
 uint64_t bw_pr_fused_acc_len(int32_t iterations);

@@ -1,0 +1,123 @@
+"""The binned PageRank iteration (propagation blocking) on the GPU.
+
+Ranks against the oracle: the FP32 restatement of the reference's
+vertex-pull-nodiv (pagerank/vertex_pull_nodiv.cu:3-29 with
+consolidateRankNoDiv, pagerank/pagerank.cu:142-165) and the FP64
+recurrence.  Bar: 1e-6 max-abs against the FP32 oracle, widened by the
+FP32 oracle's own error against FP64 (its sequential sums drift on hub
+rows), and 1e-6 against the FP64 recurrence; the FP64 row accumulators
+add in a run-dependent order, so ranks repeat to FP32 rounding.
+"""
+import numpy as np
+import pytest
+
+from belewitte_amd import device, graph, pagerank
+from oracle import binding as O
+from test_degenerate_gpu import degenerate_graphs
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-6
+SETTINGS = [(13, 1, 8), (13, 64, 16), (14, 3, 8), (14, 64, 16)]
+
+
+def _hub_graph():
+    r = graph.rmat(15, 4, seed=41)
+    s, d = r.edges()
+    hub_src = np.arange(1, 20001, dtype=np.uint32)
+    return graph.from_edges(r.vertex_count, np.concatenate([s, hub_src]),
+                            np.concatenate([d, np.zeros(hub_src.size, dtype=np.uint32)]))
+
+
+@pytest.fixture(scope="module")
+def cases(gpu):
+    out = []
+    for g in (graph.rmat(12, 16, seed=31), graph.erdos_renyi(2000, 9000, seed=3, undirected=False),
+              graph.rmat(11, 8, seed=32, undirected=True), graph.star(300), graph.chain(50), _hub_graph()):
+        out.append((g, O.CSR.from_graph(g), device.DeviceGraph(g)))
+    yield out
+    for _, _, dg in out:
+        dg.free()
+
+
+def _check(g, csr, ranks, resid, mode, quirk, iters):
+    """Within 1e-6 of the FP32 oracle widened by its own error (the
+    reference's sequential FP32 sums; in DROP mode without the undivided
+    first iteration its vertex-pull variant, which is the same
+    recurrence), within 1e-6 of the FP64 recurrence, and never less
+    accurate than the FP32 oracle."""
+    x = O.pagerank_f64(csr, iters, mode, nodiv_quirk=quirk)
+    variant = "vertex-pull-nodiv" if (mode == "uniform" or quirk) else "vertex-pull"
+    want, _ = O.pagerank_f32(csr, pagerank.variant_id(variant), iters, mode)
+    ref_err = float(np.max(np.abs(want.astype(np.float64) - x)))
+    err = float(np.max(np.abs(ranks.astype(np.float64) - want)))
+    assert err <= TOL + ref_err, (g.name, mode, quirk, err, ref_err)
+    err64 = float(np.max(np.abs(ranks - x)))
+    assert err64 <= TOL and err64 <= ref_err + 1e-7, (g.name, mode, quirk, err64, ref_err)
+    assert np.all(np.isfinite(resid))
+
+
+@pytest.mark.parametrize("rows_log2,chunk_tasks,lanes", SETTINGS)
+@pytest.mark.parametrize("mode,quirk,iters", [("uniform", False, 30), ("drop", True, 100), ("drop", False, 20)])
+def test_binned_matches_oracle(cases, rows_log2, chunk_tasks, lanes, mode, quirk, iters):
+    for g, csr, dg in cases:
+        print(f"case {g.name} binned {rows_log2}/{chunk_tasks}/{lanes} mode={mode} quirk={quirk}", flush=True)
+        fp = pagerank.FusedPageRank(dg, max_iterations=iters)
+        pull, rp, _ = fp.run(iters, mode, nodiv_quirk=quirk)
+        fp.bin(rows_log2, chunk_tasks, lanes)
+        ranks, resid, _ = fp.run(iters, mode, nodiv_quirk=quirk)
+        _check(g, csr, ranks, resid, mode, quirk, iters)
+        # The residuals follow the pull kernel's (Kahan-compensated) ones to
+        # FP32 rounding.
+        assert np.allclose(resid, rp, rtol=1e-3, atol=1e-6), (g.name, resid[:4], rp[:4])
+        again, _, _ = fp.run(iters, mode, nodiv_quirk=quirk)
+        assert float(np.max(np.abs(again - ranks))) <= 4e-7 * float(ranks.max())
+        # Back to the pull kernels: the same ranks as before, bit for bit.
+        fp.bin(0)
+        back, _, _ = fp.run(iters, mode, nodiv_quirk=quirk)
+        assert np.array_equal(back.view(np.uint32), pull.view(np.uint32))
+        fp.close()
+
+
+@pytest.mark.parametrize("rows_log2,chunk_tasks,lanes", [(13, 2, 16), (13, 1, 8), (14, 64, 8)])
+def test_binned_degree_ordered(gpu, rows_log2, chunk_tasks, lanes):
+    """Several partitions, many chunks each (R-MAT scale 16 relabelled by
+    decreasing out-degree: partition 0 holds the hubs)."""
+    g = graph.rmat(16, 16, seed=51)
+    csr = O.CSR.from_graph(g)
+    fp = pagerank.FusedPageRank.for_graph(g, max_iterations=30, reorder=True)
+    fp.tune(0, 0, 1, 2, 16)
+    pull, _, _ = fp.run(30, "uniform")
+    fp.bin(rows_log2, chunk_tasks, lanes)
+    info = fp.bin_info()
+    assert info["chunks"] > 8 and info["blocks"] >= 16 and 0 < info["pairs"] < g.edge_count
+    ranks, resid, _ = fp.run(30, "uniform")
+    _check(g, csr, ranks, resid, "uniform", False, 30)
+    assert float(np.max(np.abs(ranks - pull))) <= 2 * TOL
+    fp.close()
+
+
+@pytest.mark.parametrize("mode", ["drop", "uniform"])
+def test_binned_degenerate(gpu, mode):
+    for g in degenerate_graphs():
+        csr = O.CSR.from_graph(g)
+        dg = device.DeviceGraph(g)
+        fp = pagerank.FusedPageRank(dg, max_iterations=10)
+        fp.bin(13, 1, 8)
+        ranks, resid, _ = fp.run(10, mode)
+        x = O.pagerank_f64(csr, 10, mode)
+        assert float(np.max(np.abs(ranks - x))) <= TOL, g.name
+        assert np.all(np.isfinite(resid))
+        fp.close()
+        dg.free()
+
+
+def test_binned_rejects_bad_settings(gpu):
+    g = graph.rmat(10, 8, seed=1)
+    dg = device.DeviceGraph(g)
+    fp = pagerank.FusedPageRank(dg, max_iterations=5)
+    for args in ((12, 1, 8), (15, 1, 8), (13, 0, 8), (13, 1, 12)):
+        with pytest.raises(Exception):
+            fp.bin(*args)
+    fp.close()
+    dg.free()
