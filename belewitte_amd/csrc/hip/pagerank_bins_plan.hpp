@@ -32,10 +32,18 @@
 namespace bw {
 namespace bins {
 
+// Scatter block limits (measured on R-MAT scale 22: 2048 / 32768 beat
+// 512 / 8192, 1024 / 16384, 4096 / 65536 and 8192 / 131072).
+#ifndef BW_BINS_BLOCK_VERTS
+#define BW_BINS_BLOCK_VERTS 2048
+#endif
+#ifndef BW_BINS_BLOCK_WORK
+#define BW_BINS_BLOCK_WORK 32768
+#endif
 constexpr uint32_t kTaskEdges = 2048;    // edges of one wave task
 constexpr uint32_t kMaxRuns = 2048;      // partitions one scatter block may point into
-constexpr uint32_t kBlockVerts = 4096;   // vertices of one scatter block at most
-constexpr uint32_t kBlockWork = 65536;   // vertices x chunks of their partition + pairs, per block
+constexpr uint32_t kBlockVerts = BW_BINS_BLOCK_VERTS;  // vertices of one scatter block at most
+constexpr uint32_t kBlockWork = BW_BINS_BLOCK_WORK;    // vertices x chunks of their partition + pairs, per block
 constexpr uint16_t kNewPair = 0x8000;    // edge code flag: the edge opens a new pair
 
 struct Quad {

@@ -550,30 +550,51 @@ __global__ __launch_bounds__(kThreads, kRowsLog == 13 ? 8 : 4) void pr_bins_accu
         const uint4 ch = a.chunks[c];
         for (uint32_t i = threadIdx.x; i < kRows; i += kThreads) acc[i] = 0.0;
         __syncthreads();
-        // Rounds are software-pipelined: the codes of round r + 2 and the
-        // pair values of round r + 1 are in flight while round r adds.
-        for (uint32_t t = ch.x + wid; t < ch.y; t += kWaves) {
-            const uint4 tk = a.tasks[t];
+        // The wave's rounds, over all its tasks of the chunk (t = ch.x + wid,
+        // t + kWaves, ...), are software-pipelined: the codes of round r + 2
+        // and the pair values of round r + 1 are in flight while round r
+        // adds, also across task boundaries.
+        uint32_t t = ch.x + wid;
+        if (t < ch.y) {
+            uint4 tk = a.tasks[t];        // task of the round being staged
             uint32_t base = tk.z;
+            uint32_t le = tk.x;           // edge of the round whose codes are loaded next
+            uint32_t lt = t;              // and its task
+            uint4 ltk = tk;
             u32x4 raw[kVec];
-            uint32_t code[kLaneEdges], n;
-            float v[kLaneEdges];
-            // Decodes the round at e0 from raw, computes its pairs and
-            // issues its value loads into (code, v, n).
-            auto stage = [&](uint32_t e0, uint32_t (&cd)[kLaneEdges], float (&vv)[kLaneEdges], uint32_t &nn) {
-                const uint32_t e = e0 + lane * kLaneEdges;
+            // Loads the codes of the round at (lt, le), then steps (lt, le)
+            // to the following round; false when the wave has no round left.
+            auto load_next = [&]() {
+                if (lt >= ch.y) return false;
+                load_codes(le, ltk.y, raw);
+                le += kRound;
+                if (le >= ltk.y) {
+                    lt += kWaves;
+                    if (lt < ch.y) {
+                        ltk = a.tasks[lt];
+                        le = ltk.x;
+                    }
+                }
+                return true;
+            };
+            uint32_t se = tk.x;           // edge of the round staged next
+            // Takes `raw` as the round at se of task tk (codes kept packed,
+            // two per word), computes its pairs and issues its value loads
+            // into (cd, vv, nn).
+            auto code_of = [](const uint32_t (&cd)[kLaneEdges / 2], int j) {
+                return (cd[j >> 1] >> (16 * (j & 1))) & 0xFFFFu;
+            };
+            auto stage = [&](uint32_t (&cd)[kLaneEdges / 2], float (&vv)[kLaneEdges], uint32_t &nn) {
+                const uint32_t e = se + lane * kLaneEdges;
                 nn = e < tk.y ? min(static_cast<uint32_t>(kLaneEdges), tk.y - e) : 0u;
 #pragma unroll
                 for (int q = 0; q < kVec; q++)
 #pragma unroll
-                    for (int h = 0; h < 4; h++) {
-                        cd[8 * q + 2 * h] = raw[q][h] & 0xFFFFu;
-                        cd[8 * q + 2 * h + 1] = raw[q][h] >> 16;
-                    }
+                    for (int h = 0; h < 4; h++) cd[4 * q + h] = raw[q][h];
                 uint32_t cnt = 0;
 #pragma unroll
                 for (int j = 0; j < kLaneEdges; j++)
-                    cnt += (j < static_cast<int>(nn) && (cd[j] & bins::kNewPair)) ? 1u : 0u;
+                    cnt += (j < static_cast<int>(nn) && (code_of(cd, j) & bins::kNewPair)) ? 1u : 0u;
                 uint32_t pre = 0, tot = 0;
 #pragma unroll
                 for (int b = 0; b < kCountBits; b++) {
@@ -584,33 +605,43 @@ __global__ __launch_bounds__(kThreads, kRowsLog == 13 ? 8 : 4) void pr_bins_accu
                 uint32_t p = base + pre;
 #pragma unroll
                 for (int j = 0; j < kLaneEdges; j++) {
-                    p += (cd[j] & bins::kNewPair) ? 1u : 0u;
+                    p += (code_of(cd, j) & bins::kNewPair) ? 1u : 0u;
                     vv[j] = j < static_cast<int>(nn) ? a.bins[p] : 0.0f;
                 }
                 base += tot;
+                se += kRound;
+                if (se >= tk.y) {
+                    t += kWaves;
+                    if (t < ch.y) {
+                        tk = a.tasks[t];
+                        base = tk.z;
+                        se = tk.x;
+                    }
+                }
             };
-            load_codes(tk.x, tk.y, raw);
-            stage(tk.x, code, v, n);
-            load_codes(tk.x + kRound, tk.y, raw);
-            for (uint32_t e0 = tk.x; e0 < tk.y; e0 += kRound) {
-                uint32_t code2[kLaneEdges], n2 = 0;
+            uint32_t code[kLaneEdges / 2], n;
+            float v[kLaneEdges];
+            load_next();
+            stage(code, v, n);
+            bool more = load_next();
+            for (;;) {
+                uint32_t code2[kLaneEdges / 2], n2 = 0;
                 float v2[kLaneEdges];
-                const bool more = e0 + kRound < tk.y;  // wave-uniform
-                if (more) {
-                    stage(e0 + kRound, code2, v2, n2);
-                    load_codes(e0 + 2 * kRound, tk.y, raw);
+                const bool staged = more;  // wave-uniform
+                if (staged) {
+                    stage(code2, v2, n2);
+                    more = load_next();
                 }
 #pragma unroll
                 for (int j = 0; j < kLaneEdges; j++)
-                    if (j < static_cast<int>(n)) atomicAdd(&acc[code[j] & (kRows - 1)], static_cast<double>(v[j]));
-                if (more) {
+                    if (j < static_cast<int>(n))
+                        atomicAdd(&acc[code_of(code, j) & (kRows - 1)], static_cast<double>(v[j]));
+                if (!staged) break;  // that was the wave's last round
 #pragma unroll
-                    for (int j = 0; j < kLaneEdges; j++) {
-                        code[j] = code2[j];
-                        v[j] = v2[j];
-                    }
-                    n = n2;
-                }
+                for (int j = 0; j < kLaneEdges / 2; j++) code[j] = code2[j];
+#pragma unroll
+                for (int j = 0; j < kLaneEdges; j++) v[j] = v2[j];
+                n = n2;
             }
         }
         __syncthreads();

@@ -352,6 +352,8 @@ struct FusedPageRank : public ImplementationBase {
     bool keepOrder = false;
     int windows = 2;
     int rowsGrid = 0;
+    int binRows = 13;
+    int binChunk = 32;
     bw_graph *host = nullptr;
     bw_graph_dev dev{};
     std::vector<uint32_t> perm;  // file id -> device id, when relabelled
@@ -365,6 +367,9 @@ struct FusedPageRank : public ImplementationBase {
         options.add("no-reorder", keepOrder, true, "Keep the file's vertex order on the device.");
         options.add("windows", "NUM", windows, "Row tiles per source-sorted window (0: off, 1, 2, 4).");
         options.add("rows-grid", "NUM", rowsGrid, "Persistent row workgroups (0: 3 per CU).");
+        options.add("bins", "LOG2", binRows,
+                    "Binned iteration with 2^LOG2 rows per partition (13 or 14; 0: the pull kernels).");
+        options.add("bin-chunk", "NUM", binChunk, "Tasks of 2048 edges per chunk of the binned iteration.");
     }
 
     ~FusedPageRank() override { FusedPageRank::freeGraph(); }
@@ -425,6 +430,7 @@ struct FusedPageRank : public ImplementationBase {
         const uint32_t grid = rowsGrid > 0 ? static_cast<uint32_t>(rowsGrid) : 3u * static_cast<uint32_t>(props.compute_units);
         BW_CHK(bw_pr_plan_tune(plan, 0, grid, 1));
         BW_CHK(bw_pr_plan_sort_windows(plan, windows, 16));
+        BW_CHK(bw_pr_plan_bin(plan, binRows, binChunk, 8));
         float *ranks = static_cast<float *>(upload(nullptr, 4 * V));
         float *scratch = static_cast<float *>(upload(nullptr, 4 * V));
         double *acc = static_cast<double *>(upload(nullptr, 8 * bw_pr_fused_acc_len(max_iterations)));

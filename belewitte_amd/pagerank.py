@@ -163,29 +163,43 @@ class FusedPageRank:
         """Times each launch shape over `iterations` iterations; keeps the
         first candidate (the default shape) unless another is faster by
         more than `margin`, so that run-to-run noise does not flip the
-        choice.  Returns {shape: ms}."""
+        choice.  A shape is ("bins", rows_log2, chunk_tasks, lane_edges)
+        for the binned iteration, else the pull kernels' (hot, rows_grid,
+        tiles, sorted_windows, batch).  Returns {shape: ms}."""
         from .device import device_props
         cus = device_props(0)["compute_units"]
-        cands = candidates or [(0, 3 * cus, 1, 2, 16), (0, 8 * cus, 1, 0, 16), (0, 2 * cus, 1, 4, 8),
-                               (0, 2 * cus, 1, 4, 16), (0, 4 * cus, 1, 2, 8), (0, 4 * cus, 1, 2, 16),
-                               (0, 8 * cus, 1, 1, 8)]
+        cands = candidates or [("bins", 13, 32, 8), ("bins", 14, 64, 8), ("bins", 13, 64, 8),
+                               ("bins", 14, 32, 8), (0, 3 * cus, 1, 2, 16), (0, 2 * cus, 1, 4, 16)]
         table = {}
         for shape in cands:
-            self.tune(*shape)
+            self._apply(shape)
             self.run(iterations, "uniform", download=False)
             table[shape] = min(self.run(iterations, "uniform", download=False)[2] for _ in range(3))
         best = min(table, key=table.get)
         if table[best] > (1.0 - margin) * table[cands[0]]:
             best = cands[0]
-        self.tune(*best)
-        return {f"hot{h}/grid{g}/tiles{t}" + (f"/sorted{w}/batch{b}" if w else ""): round(v, 4)
-                for (h, g, t, w, b), v in table.items()}
+        self._apply(best)
+        return {self._shape_name(s): round(v, 4) for s, v in table.items()}
+
+    def _apply(self, shape) -> None:
+        if shape[0] == "bins":
+            self.bin(*shape[1:])
+        else:
+            self.bin(0)
+            self.tune(*shape)
+
+    @staticmethod
+    def _shape_name(shape) -> str:
+        if shape[0] == "bins":
+            return f"bins/rows{1 << shape[1]}/chunk{shape[2]}/lanes{shape[3]}"
+        h, g, t, w, b = shape
+        return f"hot{h}/grid{g}/tiles{t}" + (f"/sorted{w}/batch{b}" if w else "")
 
     def kernel_names(self) -> str:
         """The kernels one iteration launches under the current shape."""
         if self.binning:
             r, _, lanes = self.binning
-            return f"pr_bins_accumulate_kernel<{r}, 1024, {lanes}> + pr_bins_finalise_kernel<{r}, 256>"
+            return f"pr_bins_accumulate_kernel<{r}, 1024, {lanes}> + pr_bins_finalise_kernel<{r}, 512>"
         hot, _, tiles, width, batch = self.tuning
         rows = f"pr_windows_kernel<{width}, {batch}>" if width else f"pr_rows_kernel<{hot}, {tiles}>"
         return f"pr_pieces_kernel + {rows}"

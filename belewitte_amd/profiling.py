@@ -8,8 +8,10 @@
         The gather-rate probe (bw_hip_gather_probe): 64 Mi gathers from a
         float table where `share` consecutive lanes read one 128-byte line;
         prints lanes/s and distinct lines/s per sharing factor.
-    python -m belewitte_amd.profiling pagerank --scale 22 --iterations 5 [--reorder] [--grid N --hot H --tiles T] [--sorted W --batch B]
-        A few fused PageRank iterations on the benchmark graph.
+    python -m belewitte_amd.profiling pagerank --scale 22 --iterations 5 [--reorder] [--grid N --hot H --tiles T] [--sorted W --batch B] [--bins R,C,L]
+        A few fused PageRank iterations on the benchmark graph (--bins: the
+        binned iteration with 2^R rows per partition, chunks of C tasks, L
+        edges per lane).
     python -m belewitte_amd.profiling msbfs --scale 20 --sources 64 [--wave-degree W --topdown-divisor D --hub-degree H]
         One multi-source search (64 sources) on the directed R-MAT of the
         batched BFS workload.
@@ -95,7 +97,7 @@ def gather(table_mib: int, shares, mi: int = 64, threads: int = 256 * 8192, repe
 
 
 def pagerank(scale: int, iterations: int, reorder: bool, hot: int = 0, grid: int = 0, tiles: int = 1,
-             sorted_windows: int = 0, batch: int = 16) -> None:
+             sorted_windows: int = 0, batch: int = 16, bins: str = "") -> None:
     from . import device, graph, pagerank as pr
     device.require_gpu()
     device.set_device(0)
@@ -105,6 +107,8 @@ def pagerank(scale: int, iterations: int, reorder: bool, hot: int = 0, grid: int
     dg = device.DeviceGraph(g, reps=("rev_csr",))
     f = pr.FusedPageRank(dg, max_iterations=iterations)
     f.tune(hot, grid, tiles, sorted_windows, batch)
+    if bins:
+        f.bin(*(int(x) for x in bins.split(",")))
     _, _, ms = f.run(iterations, "uniform", download=False)
     print(f"pagerank: V={g.vertex_count} E={g.edge_count} {iterations} iterations {ms:.3f} ms", flush=True)
 
@@ -166,6 +170,7 @@ def main(argv=None) -> int:
     p.add_argument("--tiles", type=int, default=1)
     p.add_argument("--sorted", type=int, default=0, help="tiles per source-sorted window (0: off)")
     p.add_argument("--batch", type=int, default=16, help="gathers in flight per thread (window kernel)")
+    p.add_argument("--bins", default="", help="binned iteration: rows_log2,chunk_tasks,lane_edges")
     q = sub.add_parser("gather")
     q.add_argument("--table-mib", type=int, default=16)
     q.add_argument("--share", default="1,2,4,8,16,32,64", help="lanes per line, comma separated")
@@ -189,7 +194,7 @@ def main(argv=None) -> int:
     elif a.cmd == "msbfs":
         msbfs(a.scale, a.sources, a.wave_degree, a.topdown_divisor, a.hub_degree)
     elif a.cmd == "pagerank":
-        pagerank(a.scale, a.iterations, a.reorder, a.hot, a.grid, a.tiles, a.sorted, a.batch)
+        pagerank(a.scale, a.iterations, a.reorder, a.hot, a.grid, a.tiles, a.sorted, a.batch, a.bins)
     else:
         bfs(a.scale, a.sources, a.policy)
     return 0

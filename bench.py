@@ -43,16 +43,36 @@ def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def traffic_from_profile():
-    """HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE +
-    WRITE_SIZE, calibrated on a copy kernel); see profiles/."""
-    p = ROOT / "profiles" / "pagerank_fused_rmat_s22_traffic.json"
+def traffic_from_profile(fused):
+    """HBM bytes per iteration measured with rocprofv3 --pmc (FETCH_SIZE +
+    WRITE_SIZE, calibrated on a copy kernel) for the kernels `fused` runs,
+    from the summary under profiles/ that names the same kernels and launch
+    shape; None when there is none."""
+    name = "pagerank_binned_rmat_s22_traffic.json" if fused.binning else "pagerank_fused_rmat_s22_traffic.json"
+    p = ROOT / "profiles" / name
     if not p.exists():
         return None
     try:
-        return json.loads(p.read_text()).get("bytes_per_launch")
+        rec = json.loads(p.read_text())
     except (OSError, ValueError):
         return None
+    if fused.binning and tuple(rec.get("binning", ())) != tuple(fused.binning):
+        return None
+    return rec.get("bytes_per_launch")
+
+
+def launch_shape(fused, shapes) -> dict:
+    if fused.binning:
+        r, c, l = fused.binning
+        info = fused.bin_info()
+        shape = {"binned": True, "rows_per_partition": 1 << r, "chunk_tasks": c, "lane_edges": l,
+                 "pairs": info["pairs"], "chunks": info["chunks"], "scatter_blocks": info["blocks"]}
+    else:
+        shape = {"binned": False, "hot_vertices": fused.tuning[0], "rows_grid": fused.tuning[1],
+                 "tiles_per_workgroup": fused.tuning[2], "source_sorted_windows": fused.tuning[3],
+                 "window_gathers_per_round": fused.tuning[4]}
+    shape["autotune_ms_per_10_iterations"] = shapes
+    return shape
 
 
 def cpu_baseline(g, ranks_gpu, threads: int):
@@ -316,8 +336,7 @@ def main() -> int:
     fused = pagerank.FusedPageRank.for_graph(g, max_iterations=ITERS, reorder=True)
     log(f"degree-ordered upload + plan in {time.perf_counter() - t0:.1f}s: {fused.info()}")
     shapes = fused.autotune()
-    log(f"launch shapes (ms per 10 iterations): {shapes}; using (hot, grid, tiles, window, batch) "
-        f"{fused.tuning}: {fused.kernel_names()}")
+    log(f"launch shapes (ms per 10 iterations): {shapes}; using {fused.kernel_names()}")
 
     # Warm-up, then K timed steps bracketed by device synchronisation.
     for _ in range(args.warmup):
@@ -334,9 +353,9 @@ def main() -> int:
     iter_s = ms_per_step / 1e3 / ITERS
     value = E / iter_s / 1e9
 
-    # Roofline of one iteration (pr_pieces_kernel + pr_rows_kernel, the two
-    # launches of an iteration): algorithmic bytes 8E + 16V + 4 over the
-    # average iteration duration, HIP events on the launches' stream.
+    # Roofline of one iteration (its two launches): algorithmic bytes
+    # 8E + 16V + 4 of the pull iteration over the average iteration
+    # duration, HIP events on the launches' stream.
     bytes_per_launch = 8 * E + 16 * V + 4
     avg_launch_s = float(np.mean(kernel_ms)) / 1e3 / ITERS
     achieved = bytes_per_launch / avg_launch_s / 1e9
@@ -361,13 +380,9 @@ def main() -> int:
                                "vertices relabelled by out-degree before upload",
                    "vertices": V, "edges": E, "iterations": ITERS, "parallelism": "single-gpu"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profile(),
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profile(fused),
                      "kernel": fused.kernel_names() + " (one iteration)",
-                     "launch_shape": {"hot_vertices": fused.tuning[0], "rows_grid": fused.tuning[1],
-                                      "tiles_per_workgroup": fused.tuning[2],
-                                      "source_sorted_windows": fused.tuning[3],
-                                      "window_gathers_per_round": fused.tuning[4],
-                                      "autotune_ms_per_10_iterations": shapes},
+                     "launch_shape": launch_shape(fused, shapes),
                      "algorithmic_bytes_per_launch": bytes_per_launch,
                      "avg_launch_us": round(avg_launch_s * 1e6, 2)},
         "device": {"arch": props["arch"], "compute_units": props["compute_units"]},
@@ -389,6 +404,7 @@ def main() -> int:
     # The same fused kernel on the graph in its generated vertex order.
     dg = device.DeviceGraph(g, reps=("rev_csr",))
     natural = pagerank.FusedPageRank(dg, max_iterations=ITERS)
+    natural._apply(("bins",) + fused.binning if fused.binning else fused.tuning)
     natural.run(ITERS, "uniform", download=False)
     _, _, ms_nat = natural.run(ITERS, "uniform", download=False)
     out["natural_order"] = {"ge_s_iter": round(E / (ms_nat / 1e3 / ITERS) / 1e9, 3),

@@ -172,14 +172,17 @@ def test_native_engine_through_the_runner(gfile, tmp_path):
 
 def test_fused_pagerank_through_the_runner(gfile, tmp_path):
     """vertex-pull-nodiv-fused computes vertex-pull-nodiv: DROP mode with the
-    reference's undivided first iteration; its compensated sums are never
-    less accurate than the reference's sequential FP32 ones, so the bar is
-    1e-6 plus that path's own error against the FP64 recurrence."""
+    reference's undivided first iteration; its sums (compensated in the pull
+    kernels, FP64 in the binned iteration, the default) are never less
+    accurate than the reference's sequential FP32 ones, so the bar is 1e-6
+    plus that path's own error against the FP64 recurrence."""
     g, p = gfile
     csr = O.CSR.from_graph(g)
     vid = pagerank.variant_id("vertex-pull-nodiv")
     cases = [([], 100, "drop"), (["--dangling", "uniform", "--iterations", "30"], 30, "uniform"),
-             (["--no-reorder", "--windows", "0"], 100, "drop"), (["--windows", "4", "--rows-grid", "7"], 100, "drop")]
+             (["--no-reorder", "--bins", "0", "--windows", "0"], 100, "drop"),
+             (["--bins", "0", "--windows", "4", "--rows-grid", "7"], 100, "drop"),
+             (["--bins", "14", "--bin-chunk", "1"], 100, "drop")]
     for extra, iters, mode in cases:
         run(["-a", "pagerank", "-k", "vertex-pull-nodiv-fused", "--validate", *extra, "-o", str(tmp_path), str(p)])
         vals = np.array([float(l.split("\t")[1]) for l in (tmp_path / "rmat11.output").read_text().splitlines()])
