@@ -18,7 +18,9 @@
 //      order and finalised into new ranks; the new contributions of the
 //      block's vertices are staged in LDS and written to every bin slot of
 //      the block's pairs -- a set of runs, one per partition the block's
-//      vertices point into, each contiguous in the bin array.
+//      vertices point into, each contiguous in the bin array.  The pairs'
+//      sources are stored in this scatter order ("flat" order), so the
+//      scatter reads them sequentially.
 // Pairs deduplicate the edges a source has into one partition, so the bin
 // traffic is 4 bytes per pair written and read instead of one gathered
 // cache line per few edges.
@@ -40,7 +42,8 @@ namespace bins {
 #ifndef BW_BINS_BLOCK_WORK
 #define BW_BINS_BLOCK_WORK 32768
 #endif
-constexpr uint32_t kTaskEdges = 2048;    // edges of one wave task
+constexpr uint32_t kTaskEdges = 2048;    // unit of the chunk size (chunk_tasks x kTaskEdges edges at most)
+constexpr uint32_t kChunkWaves = 16;     // tasks per chunk: one per wave of the accumulate workgroup
 constexpr uint32_t kMaxRuns = 2048;      // partitions one scatter block may point into
 constexpr uint32_t kBlockVerts = BW_BINS_BLOCK_VERTS;  // vertices of one scatter block at most
 constexpr uint32_t kBlockWork = BW_BINS_BLOCK_WORK;    // vertices x chunks of their partition + pairs, per block
@@ -58,14 +61,17 @@ struct Plan {
                                          // (row - partition base) | kNewPair on a pair's first edge
     std::vector<uint32_t> edge_start;    // K + 1: padded start of each partition's edges
     std::vector<uint32_t> pair_start;    // K + 1: first pair of each partition
-    std::vector<Quad> tasks;             // (edge begin, edge end, pair base, partition):
+    std::vector<Quad> tasks;             // (edge begin, edge end, pair base, partition), 8-aligned begin:
                                          // pair(e) = base + #flags in [begin, e]
-    std::vector<Quad> chunks;            // (task begin, task end, partition, slot), longest first
+    std::vector<Quad> chunks;            // (task begin, task end, partition, slot), longest first;
+                                         // a partition's edges split evenly into chunks, a chunk's
+                                         // evenly into kChunkWaves tasks
     std::vector<uint32_t> chunk_base;    // K + 1: slots of partition k are [chunk_base[k], chunk_base[k+1])
     std::vector<Quad> blocks;            // (first vertex, end vertex, run begin, run count)
-    std::vector<uint32_t> run_flat;      // per block: flat start of each run, then the block's pair total
+    std::vector<uint32_t> run_flat;      // per block: flat index of each run's first pair, then the block's end
     std::vector<uint32_t> run_bin;       // per block: bin index of each run's first pair, then 0
-    std::vector<uint16_t> psrc;          // per pair (bin order): source - first vertex of its block
+    std::vector<uint16_t> psrc;          // per pair in flat order (blocks in vertex order, each block's
+                                         // runs in partition order): source - first vertex of its block
 };
 
 // Builds the plan from a reverse CSR (row r's in-edge sources are
@@ -150,19 +156,19 @@ inline bool build_plan(const uint32_t *off, const uint32_t *cols, uint32_t V, ui
     P.edge_start[K] = static_cast<uint32_t>(ep);
     P.pairs = static_cast<uint32_t>(u);
 
-    // Chunks per partition: ceil(tasks / chunk_tasks).
+    // Chunks per partition: ceil(edges / (chunk_tasks x kTaskEdges)).
+    const uint64_t chunkEdges = static_cast<uint64_t>(chunk_tasks) * kTaskEdges;
     P.chunk_base.assign(K + 1, 0);
-    for (uint32_t k = 0; k < K; k++) {
-        const uint32_t ntasks = (edgesK[k] + kTaskEdges - 1) / kTaskEdges;
-        P.chunk_base[k + 1] = P.chunk_base[k] + (ntasks + chunk_tasks - 1) / chunk_tasks;
-    }
+    for (uint32_t k = 0; k < K; k++)
+        P.chunk_base[k + 1] = P.chunk_base[k] + static_cast<uint32_t>((edgesK[k] + chunkEdges - 1) / chunkEdges);
     P.slots = P.chunk_base[K];
 
     // Scatter blocks: consecutive vertices of one partition, cut so that
     // vertices x chunks of the partition + pairs stays within kBlockWork,
     // at most kBlockVerts vertices and kMaxRuns runs.
     std::vector<uint32_t> stamp(K, UINT32_MAX), blen(K, 0), done(K, 0), touched;
-    std::vector<uint32_t> blockOf(V);
+    std::vector<uint32_t> blockOf(V), runPart;
+    uint32_t flat = 0;
     for (uint32_t k = 0; k < K; k++) {
         const uint32_t vEnd = std::min<uint64_t>((static_cast<uint64_t>(k) << L) + D, V);
         const uint64_t perVertex = std::max<uint32_t>(1, P.chunk_base[k + 1] - P.chunk_base[k]);
@@ -198,54 +204,69 @@ inline bool build_plan(const uint32_t *off, const uint32_t *cols, uint32_t V, ui
             }
             std::sort(touched.begin(), touched.end());
             const uint32_t runBegin = static_cast<uint32_t>(P.run_flat.size());
-            uint32_t flat = 0;
             for (uint32_t kk : touched) {
                 P.run_flat.push_back(flat);
                 P.run_bin.push_back(P.pair_start[kk] + done[kk]);
+                runPart.push_back(kk);
                 flat += blen[kk];
                 done[kk] += blen[kk];
             }
             P.run_flat.push_back(flat);
             P.run_bin.push_back(0);
+            runPart.push_back(UINT32_MAX);
             P.blocks.push_back({v0, s, runBegin, static_cast<uint32_t>(touched.size())});
         }
     }
 
-    // Edge codes and pair sources, in (partition, source, row) order.
+    // Edge codes in (partition, source, row) order; pair sources in flat
+    // order (cursor[k]: next flat slot of the current block's run into k).
+    if (flat != P.pairs) {
+        err = "internal error: pair count";
+        return false;
+    }
     P.codes.assign(std::max<uint64_t>(ep, 8), 0);
     P.psrc.assign(std::max<uint32_t>(P.pairs, 1), 0);
     {
-        std::vector<uint32_t> ec(K, 0), pc(K, 0);
+        std::vector<uint32_t> ec(K, 0), cursor(K, 0);
+        uint32_t cur = UINT32_MAX;
         for (uint32_t s = 0; s < V; s++) {
-            const uint32_t v0 = P.blocks[blockOf[s]].x;
+            const Quad &b = P.blocks[blockOf[s]];
+            if (blockOf[s] != cur) {
+                cur = blockOf[s];
+                for (uint32_t i = b.z; i < b.z + b.w; i++) cursor[runPart[i]] = P.run_flat[i];
+            }
             uint32_t prev = UINT32_MAX;
             for (uint32_t e = foff[s]; e < foff[s + 1]; e++) {
                 const uint32_t r = frow[e], k = r >> L;
                 const bool fresh = k != prev;
                 prev = k;
-                if (fresh) P.psrc[P.pair_start[k] + pc[k]++] = static_cast<uint16_t>(s - v0);
+                if (fresh) P.psrc[cursor[k]++] = static_cast<uint16_t>(s - b.x);
                 P.codes[P.edge_start[k] + ec[k]++] = static_cast<uint16_t>((r & (D - 1)) | (fresh ? kNewPair : 0));
             }
         }
     }
 
-    // Wave tasks of kTaskEdges edges, and the chunks over them.
+    // Chunks: each partition's edges split evenly (boundaries rounded down
+    // to 8 edges), each chunk's evenly into kChunkWaves tasks (empty ones
+    // dropped).
     for (uint32_t k = 0; k < K; k++) {
-        const uint32_t t0 = static_cast<uint32_t>(P.tasks.size());
-        uint32_t flags = 0;
-        for (uint32_t b = 0; b < edgesK[k]; b += kTaskEdges) {
-            const uint32_t n = std::min(kTaskEdges, edgesK[k] - b);
-            const uint32_t eb = P.edge_start[k] + b;
-            P.tasks.push_back({eb, eb + n, P.pair_start[k] - 1u + flags, k});
-            for (uint32_t i = 0; i < n; i++) flags += (P.codes[eb + i] & kNewPair) ? 1u : 0u;
-        }
-        const uint32_t t1 = static_cast<uint32_t>(P.tasks.size());
-        uint32_t slot = P.chunk_base[k];
-        for (uint32_t t = t0; t < t1; t += chunk_tasks)
-            P.chunks.push_back({t, std::min(t1, t + chunk_tasks), k, slot++});
-        if (slot != P.chunk_base[k + 1]) {
-            err = "internal error: chunk count";
-            return false;
+        const uint64_t n = edgesK[k], nch = P.chunk_base[k + 1] - P.chunk_base[k];
+        const uint32_t es = P.edge_start[k];
+        uint32_t flags = 0, counted = 0;  // new-pair flags in [es, es + counted)
+        auto flags_before = [&](uint32_t off) {
+            for (; counted < off; counted++) flags += (P.codes[es + counted] & kNewPair) ? 1u : 0u;
+            return flags;
+        };
+        auto round8 = [](uint64_t x) { return static_cast<uint32_t>(x & ~7ull); };
+        for (uint64_t c = 0; c < nch; c++) {
+            const uint32_t cb = round8(n * c / nch), ce = c + 1 == nch ? static_cast<uint32_t>(n) : round8(n * (c + 1) / nch);
+            const uint32_t t0 = static_cast<uint32_t>(P.tasks.size());
+            for (uint32_t w = 0; w < kChunkWaves; w++) {
+                const uint32_t tb = cb + round8(static_cast<uint64_t>(ce - cb) * w / kChunkWaves);
+                const uint32_t te = w + 1 == kChunkWaves ? ce : cb + round8(static_cast<uint64_t>(ce - cb) * (w + 1) / kChunkWaves);
+                if (te > tb) P.tasks.push_back({es + tb, es + te, P.pair_start[k] - 1u + flags_before(tb), k});
+            }
+            P.chunks.push_back({t0, static_cast<uint32_t>(P.tasks.size()), k, P.chunk_base[k] + static_cast<uint32_t>(c)});
         }
     }
     std::stable_sort(P.chunks.begin(), P.chunks.end(),

@@ -479,12 +479,15 @@ __global__ __launch_bounds__(kBlock * kWinTiles) void pr_windows_kernel(IterArgs
 
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
+
 struct BinArgs {
     uint32_t V;
     const uint16_t *codes;      // per edge: local row | new-pair flag
     const uint4 *tasks;         // (edge begin, edge end, pair base, partition)
     const uint4 *chunks;        // (task begin, task end, partition, slot)
     uint32_t nchunks;
+    uint32_t ncodes;            // length of codes (a multiple of 8, at least 8)
+    uint32_t npairs;            // length of bins (at least 1)
     float *bins;                // one contribution per pair
     float *partials;            // [slot][rows per partition]
     const uint32_t *chunk_base; // per partition: its first slot (K + 1)
@@ -513,7 +516,7 @@ struct BinArgs {
 // round's already in flight), finds the pair of each edge from the
 // new-pair flags (wave prefix over per-lane flag counts, by ballots of
 // their bits), reads the pairs' values and adds each to its row.
-template<int kRowsLog, int kThreads, int kLaneEdges>
+template<int kRowsLog, int kThreads, int kLaneEdges, int kDepth>
 __global__ __launch_bounds__(kThreads, kRowsLog == 13 ? 8 : 4) void pr_bins_accumulate_kernel(BinArgs a)
 {
     constexpr uint32_t kRows = 1u << kRowsLog;
@@ -521,8 +524,9 @@ __global__ __launch_bounds__(kThreads, kRowsLog == 13 ? 8 : 4) void pr_bins_accu
     constexpr int kVec = kLaneEdges / 8;                 // 16-byte loads per lane and round
     constexpr uint32_t kRound = kWave * kLaneEdges;      // edges per wave and round
     constexpr int kCountBits = kLaneEdges == 8 ? 4 : 5;  // flag count 0..kLaneEdges
-    static_assert(bins::kTaskEdges % kRound == 0, "a task is whole rounds");
+    static_assert(kWaves == static_cast<int>(bins::kChunkWaves), "the plan makes one task per wave");
     __shared__ double acc[kRows];
+    __shared__ uint4 stask[bins::kChunkWaves];  // the chunk's task records
     __shared__ double red[kWaves][2];
     if (blockIdx.x == gridDim.x - 1) {
         double r, g;
@@ -536,42 +540,48 @@ __global__ __launch_bounds__(kThreads, kRowsLog == 13 ? 8 : 4) void pr_bins_accu
     const int lane = threadIdx.x & (kWave - 1);
     const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
     const uint64_t below = (1ull << lane) - 1ull;
-    // The lane's codes of the round starting at edge e0 (zeros past end;
-    // every 8-edge group starts 8-aligned inside the padded partition).
-    auto load_codes = [&](uint32_t e0, uint32_t end, u32x4 (&raw)[kVec]) {
+    // The lane's codes of the round starting at edge e0.  The loads are
+    // unconditional (no branch, so the compiler can keep several rounds in
+    // flight): a group past the end of the codes is clamped to their last
+    // 8, and codes past the task's end are masked by the edge count later.
+    auto load_codes = [&](uint32_t e0, u32x4 (&raw)[kVec]) {
 #pragma unroll
         for (int q = 0; q < kVec; q++) {
-            const uint32_t e = e0 + lane * kLaneEdges + 8 * q;
-            raw[q] = u32x4{0u, 0u, 0u, 0u};
-            if (e < end) raw[q] = __builtin_nontemporal_load(reinterpret_cast<const u32x4 *>(a.codes + e));
+            const uint32_t e = min(e0 + lane * kLaneEdges + 8 * q, a.ncodes - 8);
+            raw[q] = __builtin_nontemporal_load(reinterpret_cast<const u32x4 *>(a.codes + e));
         }
     };
     for (uint32_t c = blockIdx.x; c < a.nchunks; c += gridDim.x - 1) {
         const uint4 ch = a.chunks[c];
         for (uint32_t i = threadIdx.x; i < kRows; i += kThreads) acc[i] = 0.0;
+        // Task records come from LDS in the loop: a global load there would
+        // make every wait for it a wait for all the loads in flight.
+        for (uint32_t i = threadIdx.x; i < ch.y - ch.x; i += kThreads) stask[i] = a.tasks[ch.x + i];
         __syncthreads();
         // The wave's rounds, over all its tasks of the chunk (t = ch.x + wid,
         // t + kWaves, ...), are software-pipelined: the codes of round r + 2
         // and the pair values of round r + 1 are in flight while round r
         // adds, also across task boundaries.
-        uint32_t t = ch.x + wid;
+        uint32_t t = __builtin_amdgcn_readfirstlane(ch.x + wid);
         if (t < ch.y) {
-            uint4 tk = a.tasks[t];        // task of the round being staged
+            uint4 tk = stask[t - ch.x];   // task of the round being staged
             uint32_t base = tk.z;
             uint32_t le = tk.x;           // edge of the round whose codes are loaded next
             uint32_t lt = t;              // and its task
             uint4 ltk = tk;
-            u32x4 raw[kVec];
-            // Loads the codes of the round at (lt, le), then steps (lt, le)
-            // to the following round; false when the wave has no round left.
-            auto load_next = [&]() {
+            // Loads the codes of the round at (lt, le) into raw, then steps
+            // (lt, le) to the following round; false when the wave had no
+            // round left.  The load is unconditional (a clamped, unused one
+            // at the end), so `raw` never merges an old value with a load in
+            // flight.
+            auto load_next = [&](u32x4 (&raw)[kVec]) {
+                load_codes(le, raw);
                 if (lt >= ch.y) return false;
-                load_codes(le, ltk.y, raw);
                 le += kRound;
                 if (le >= ltk.y) {
-                    lt += kWaves;
+                    lt = __builtin_amdgcn_readfirstlane(lt + kWaves);
                     if (lt < ch.y) {
-                        ltk = a.tasks[lt];
+                        ltk = stask[lt - ch.x];
                         le = ltk.x;
                     }
                 }
@@ -584,7 +594,8 @@ __global__ __launch_bounds__(kThreads, kRowsLog == 13 ? 8 : 4) void pr_bins_accu
             auto code_of = [](const uint32_t (&cd)[kLaneEdges / 2], int j) {
                 return (cd[j >> 1] >> (16 * (j & 1))) & 0xFFFFu;
             };
-            auto stage = [&](uint32_t (&cd)[kLaneEdges / 2], float (&vv)[kLaneEdges], uint32_t &nn) {
+            auto stage = [&](const u32x4 (&raw)[kVec], uint32_t (&cd)[kLaneEdges / 2], float (&vv)[kLaneEdges],
+                             uint32_t &nn) {
                 const uint32_t e = se + lane * kLaneEdges;
                 nn = e < tk.y ? min(static_cast<uint32_t>(kLaneEdges), tk.y - e) : 0u;
 #pragma unroll
@@ -606,43 +617,67 @@ __global__ __launch_bounds__(kThreads, kRowsLog == 13 ? 8 : 4) void pr_bins_accu
 #pragma unroll
                 for (int j = 0; j < kLaneEdges; j++) {
                     p += (code_of(cd, j) & bins::kNewPair) ? 1u : 0u;
-                    vv[j] = j < static_cast<int>(nn) ? a.bins[p] : 0.0f;
+                    vv[j] = a.bins[min(p, a.npairs - 1)];  // unconditional; masked when added
                 }
                 base += tot;
                 se += kRound;
                 if (se >= tk.y) {
-                    t += kWaves;
+                    t = __builtin_amdgcn_readfirstlane(t + kWaves);
                     if (t < ch.y) {
-                        tk = a.tasks[t];
+                        tk = stask[t - ch.x];
                         base = tk.z;
                         se = tk.x;
                     }
                 }
             };
-            uint32_t code[kLaneEdges / 2], n;
-            float v[kLaneEdges];
-            load_next();
-            stage(code, v, n);
-            bool more = load_next();
-            for (;;) {
-                uint32_t code2[kLaneEdges / 2], n2 = 0;
-                float v2[kLaneEdges];
-                const bool staged = more;  // wave-uniform
-                if (staged) {
-                    stage(code2, v2, n2);
-                    more = load_next();
-                }
+            // Adds a staged round.  Masked edges add +0.0 (to some row of the
+            // partition), which leaves every sum unchanged: contributions are
+            // never negative.  The mask is applied here, a round after the
+            // loads, so that waiting for them never waits for the next
+            // round's.
+            auto add = [&](const uint32_t (&cd)[kLaneEdges / 2], const float (&vv)[kLaneEdges], uint32_t nn) {
 #pragma unroll
                 for (int j = 0; j < kLaneEdges; j++)
-                    if (j < static_cast<int>(n))
-                        atomicAdd(&acc[code_of(code, j) & (kRows - 1)], static_cast<double>(v[j]));
-                if (!staged) break;  // that was the wave's last round
+                    atomicAdd(&acc[code_of(cd, j) & (kRows - 1)],
+                              j < static_cast<int>(nn) ? static_cast<double>(vv[j]) : 0.0);
+            };
+            // kDepth rounds of pair values in flight.  Rounds R_0, R_1, ... of
+            // the wave; per step r: the codes of R_{r+kDepth+1} are loaded,
+            // R_r is added, and R_{r+kDepth} is staged into R_r's buffers from
+            // codes loaded a step earlier (issued before the values of
+            // R_{r+kDepth-1}, so waiting for them leaves those in flight).
+            // Buffers are used in turn with compile-time indices (the loop is
+            // unrolled over 2 x kDepth steps), so no register holding a load in
+            // flight is ever copied, which would make the compiler wait for
+            // it.  Stages and code loads run unconditionally: past the wave's
+            // last round they see no edge (counts 0, loads clamped), so the
+            // compiler's count of loads in flight is the same on every path.
+            static_assert(kDepth >= 2, "the pipeline keeps at least two rounds in flight");
+            u32x4 raw[2][kVec];
+            uint32_t code[kDepth][kLaneEdges / 2], n[kDepth];
+            float v[kDepth][kLaneEdges];
+            bool real[kDepth];      // wave-uniform: R_r of the buffer exists
+            bool next_real = true;  // the round whose codes were loaded last exists
+            load_next(raw[0]);
 #pragma unroll
-                for (int j = 0; j < kLaneEdges / 2; j++) code[j] = code2[j];
-#pragma unroll
-                for (int j = 0; j < kLaneEdges; j++) v[j] = v2[j];
-                n = n2;
+            for (int i = 0; i < kDepth; i++) {
+                real[i] = next_real;
+                next_real = load_next(raw[(i + 1) & 1]);
+                stage(raw[i & 1], code[i], v[i], n[i]);
             }
+            for (;;) {
+#pragma unroll
+                for (int i = 0; i < 2 * kDepth; i++) {
+                    const int cur = i % kDepth;
+                    const bool have = load_next(raw[(i + kDepth + 1) & 1]);
+                    add(code[cur], v[cur], n[cur]);
+                    if (!real[(i + 1) % kDepth]) goto wave_done;  // rounds are consecutive
+                    stage(raw[(i + kDepth) & 1], code[cur], v[cur], n[cur]);
+                    real[cur] = next_real;
+                    next_real = have;
+                }
+            }
+        wave_done:;
         }
         __syncthreads();
         float *out = a.partials + static_cast<size_t>(ch.w) * kRows;
@@ -722,10 +757,11 @@ __global__ __launch_bounds__(kThreads) void pr_bins_finalise_kernel(BinArgs a)
             __syncthreads();
             // Wave w writes the flat range [fbeg, fend) of the block's pairs;
             // each lane finds the run of its first flat by binary search,
-            // then follows the runs forward (its flats only increase).
-            const uint32_t total = rflat[bl.w];
+            // then follows the runs forward (its flats only increase).  The
+            // pairs' sources are stored in flat order: sequential reads.
+            const uint32_t f0 = rflat[0], total = rflat[bl.w] - f0;
             const uint32_t per = ((total + kWaves - 1) / kWaves + kWave - 1) & ~static_cast<uint32_t>(kWave - 1);
-            const uint32_t fbeg = min(wid * per, total), fend = min(fbeg + per, total);
+            const uint32_t fbeg = f0 + min(wid * per, total), fend = min(fbeg + per, f0 + total);
             uint32_t lo = 0;
             if (fbeg + lane < fend) {
                 const uint32_t f = fbeg + lane;
@@ -736,20 +772,20 @@ __global__ __launch_bounds__(kThreads) void pr_bins_finalise_kernel(BinArgs a)
                     else hi = mid;
                 }
             }
-            for (uint32_t f0 = fbeg; f0 < fend; f0 += kWave * kScatterUnroll) {
+            for (uint32_t fr = fbeg; fr < fend; fr += kWave * kScatterUnroll) {
                 uint32_t p[kScatterUnroll];
                 uint16_t src[kScatterUnroll];
 #pragma unroll
                 for (int u = 0; u < kScatterUnroll; u++) {
-                    const uint32_t f = f0 + lane + u * kWave;
+                    const uint32_t f = fr + lane + u * kWave;
                     p[u] = 0xFFFFFFFFu;
+                    src[u] = 0;
                     if (f < fend) {
+                        src[u] = a.psrc[f];
                         while (rflat[lo + 1] <= f) lo++;
                         p[u] = rbin[lo] + (f - rflat[lo]);
                     }
                 }
-#pragma unroll
-                for (int u = 0; u < kScatterUnroll; u++) src[u] = p[u] != 0xFFFFFFFFu ? a.psrc[p[u]] : 0;
 #pragma unroll
                 for (int u = 0; u < kScatterUnroll; u++)
                     if (p[u] != 0xFFFFFFFFu) a.bins[p[u]] = contrib[src[u]];
@@ -820,7 +856,7 @@ struct bw_pr_plan {
     const void *roff = nullptr;  // the graph's reverse offsets (for building the bins)
     // Binned mode (bw_pr_plan_bin): 2^bin_log2 rows per partition, 0 = off.
     struct Bins {
-        uint32_t log2 = 0, chunk_tasks = 0, nchunks = 0, nblocks = 0, pairs = 0, slots = 0, K = 0;
+        uint32_t log2 = 0, chunk_tasks = 0, nchunks = 0, nblocks = 0, pairs = 0, slots = 0, K = 0, ncodes = 0;
         uint32_t lane_edges = 8;  // edge codes per lane and round of the accumulate kernel
         uint16_t *codes = nullptr;
         uint4 *tasks = nullptr, *chunks = nullptr, *blocks = nullptr;
@@ -1053,7 +1089,7 @@ extern "C" int bw_pr_plan_bin(bw_pr_plan *plan, int32_t rows_log2, int32_t chunk
         return BW_HIP_OK;
     }
     if (rows_log2 != 13 && rows_log2 != 14) return fail(BW_HIP_ERR_ARG, "rows_log2 must be 0, 13 or 14");
-    if (chunk_tasks < 1 || chunk_tasks > 65536) return fail(BW_HIP_ERR_ARG, "chunk_tasks must be in [1, 65536]");
+    if (chunk_tasks < 1 || chunk_tasks > 4096) return fail(BW_HIP_ERR_ARG, "chunk_tasks must be in [1, 4096]");
     if (lane_edges != 8 && lane_edges != 16) return fail(BW_HIP_ERR_ARG, "lane_edges must be 8 or 16");
     auto &B = plan->bins;
     if (B.log2 == static_cast<uint32_t>(rows_log2) && B.chunk_tasks == static_cast<uint32_t>(chunk_tasks)) {
@@ -1106,6 +1142,7 @@ extern "C" int bw_pr_plan_bin(bw_pr_plan *plan, int32_t rows_log2, int32_t chunk
     B.nchunks = static_cast<uint32_t>(P.chunks.size());
     B.nblocks = static_cast<uint32_t>(P.blocks.size());
     B.pairs = P.pairs;
+    B.ncodes = static_cast<uint32_t>(P.codes.size());
     B.slots = P.slots;
     B.K = P.K;
     plan->binned = true;
@@ -1137,6 +1174,9 @@ extern "C" uint64_t bw_pr_fused_acc_len(int32_t iterations)
 namespace {
 
 constexpr int kBinAccThreads = 1024;
+#ifndef BW_BINS_DEPTH14
+#define BW_BINS_DEPTH14 2
+#endif
 constexpr int kBinFinThreads = 512;
 
 // The binned iteration loop: initial ranks and bins, then per iteration one
@@ -1147,10 +1187,10 @@ int run_binned(const bw_pr_plan *plan, int32_t iterations, int32_t mode, int32_t
     const auto &B = plan->bins;
     using Kernel = void (*)(BinArgs);
     const bool l16 = B.lane_edges == 16;
-    const Kernel accK = B.log2 == 13 ? (l16 ? pr_bins_accumulate_kernel<13, kBinAccThreads, 16>
-                                            : pr_bins_accumulate_kernel<13, kBinAccThreads, 8>)
-                                     : (l16 ? pr_bins_accumulate_kernel<14, kBinAccThreads, 16>
-                                            : pr_bins_accumulate_kernel<14, kBinAccThreads, 8>);
+    const Kernel accK = B.log2 == 13 ? (l16 ? pr_bins_accumulate_kernel<13, kBinAccThreads, 16, 2>
+                                            : pr_bins_accumulate_kernel<13, kBinAccThreads, 8, 2>)
+                                     : (l16 ? pr_bins_accumulate_kernel<14, kBinAccThreads, 16, 2>
+                                            : pr_bins_accumulate_kernel<14, kBinAccThreads, 8, BW_BINS_DEPTH14>);
     const Kernel finK = B.log2 == 13 ? pr_bins_finalise_kernel<13, kBinFinThreads>
                                      : pr_bins_finalise_kernel<14, kBinFinThreads>;
     BinArgs a;
@@ -1159,6 +1199,8 @@ int run_binned(const bw_pr_plan *plan, int32_t iterations, int32_t mode, int32_t
     a.tasks = B.tasks;
     a.chunks = B.chunks;
     a.nchunks = B.nchunks;
+    a.ncodes = B.ncodes;
+    a.npairs = std::max<uint32_t>(B.pairs, 1);
     a.bins = B.vals;
     a.partials = B.partials;
     a.chunk_base = B.chunk_base;

@@ -420,8 +420,9 @@ int bw_pr_plan_sort_windows(bw_pr_plan *plan, int32_t tiles_per_window,
  * partitions of 2^rows_log2; each distinct (source, partition) pair holds
  * one contribution per iteration, written once by the source and read once
  * by the partition, whose rows accumulate in LDS in FP64; a partition's
- * edges are cut into chunks of chunk_tasks x 2048 edges, one workgroup and
- * one partial row vector each; lane_edges (8 or 16) edge codes per lane and
+ * edges are split evenly into chunks of at most chunk_tasks (1 to 4096) x
+ * 2048 edges, one workgroup and one partial row vector each, a chunk's
+ * evenly over the workgroup's 16 waves; lane_edges (8 or 16) edge codes per lane and
  * round.  Same recurrence and modes as the pull kernels
  * (pagerank/vertex_pull_nodiv.cu:3-29, pagerank/pagerank.cu:142-165); the
  * LDS additions run in a run-dependent order, but each partial is an FP64

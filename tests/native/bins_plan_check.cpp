@@ -55,15 +55,20 @@ int main(int argc, char **argv)
         return 4;
     }
     const uint32_t D = 1u << L;
+    const uint32_t V_ = V;
+    (void)V_;
 
     // Coverage and bounds.
     std::vector<int> vseen(V, 0), pseen(P.pairs, 0), tseen(P.tasks.size(), 0), sseen(P.slots, 0);
+    uint32_t flatNext = 0;
     for (const Quad &b : P.blocks) {
         CHECK(b.x < b.y && b.y <= V && b.y - b.x <= kBlockVerts);
         CHECK((b.x >> L) == ((b.y - 1) >> L));
         CHECK(b.w <= kMaxRuns && static_cast<size_t>(b.z) + b.w < P.run_flat.size());
         for (uint32_t v = b.x; v < b.y; v++) vseen[v]++;
-        CHECK(P.run_flat[b.z] == 0 || b.w == 0);
+        CHECK(P.run_flat[b.z] == flatNext);  // blocks' flat ranges follow each other
+        flatNext = P.run_flat[b.z + b.w];
+        CHECK(flatNext <= P.pairs);
         for (uint32_t i = 0; i < b.w; i++) {
             const uint32_t n = P.run_flat[b.z + i + 1] - P.run_flat[b.z + i];
             CHECK(n > 0);
@@ -71,10 +76,11 @@ int main(int argc, char **argv)
                 const uint32_t p = P.run_bin[b.z + i] + j;
                 CHECK(p < P.pairs);
                 pseen[p]++;
-                CHECK(P.psrc[p] < b.y - b.x);
+                CHECK(P.psrc[P.run_flat[b.z + i] + j] < b.y - b.x);
             }
         }
     }
+    CHECK(flatNext == P.pairs);
     for (uint32_t v = 0; v < V; v++) CHECK(vseen[v] == 1);
     for (uint32_t p = 0; p < P.pairs; p++) CHECK(pseen[p] == 1);
     for (const Quad &c : P.chunks) {
@@ -91,6 +97,20 @@ int main(int argc, char **argv)
     }
     for (int x : tseen) CHECK(x == 1);
     for (int x : sseen) CHECK(x == 1);
+    // The tasks of each partition cover its edges exactly once, and a chunk
+    // has at most one task per wave of the accumulate workgroup.
+    {
+        std::vector<uint8_t> eseen(P.codes.size(), 0);
+        for (const Quad &tk : P.tasks)
+            for (uint32_t e = tk.x; e < tk.y; e++) eseen[e]++;
+        for (uint32_t k = 0; k < P.K; k++) {
+            const uint32_t r0 = k << L, r1 = std::min<uint64_t>(static_cast<uint64_t>(r0) + D, V);
+            const uint32_t n = off[r1] - off[r0];
+            for (uint32_t e = P.edge_start[k]; e < P.edge_start[k + 1]; e++)
+                CHECK(eseen[e] == (e < P.edge_start[k] + n ? 1 : 0));
+        }
+        for (const Quad &c : P.chunks) CHECK(c.y - c.x <= kChunkWaves);
+    }
 
     // The kernels' arithmetic, sequentially.
     std::vector<uint32_t> deg(V, 0);
@@ -123,14 +143,14 @@ int main(int argc, char **argv)
                 contrib[v - b.x] = cv;
             }
             if (!scatter) continue;
-            // The finalise kernel's walk: 4 waves of 64 lanes, wave w over
+            // The finalise kernel's walk: 8 waves of 64 lanes, wave w over
             // the flat range [w * per, (w + 1) * per), each lane's run found
             // by binary search for its first flat and followed forward.
             const uint32_t *rflat = &P.run_flat[b.z];
-            const uint32_t total = rflat[b.w], waves = 4;
+            const uint32_t f0 = rflat[0], total = rflat[b.w] - f0, waves = 8;
             const uint32_t per = ((total + waves - 1) / waves + 63) & ~63u;
             for (uint32_t w = 0; w < waves; w++) {
-                const uint32_t fbeg = std::min(w * per, total), fend = std::min(fbeg + per, total);
+                const uint32_t fbeg = f0 + std::min(w * per, total), fend = std::min(fbeg + per, f0 + total);
                 for (uint32_t lane = 0; lane < 64; lane++) {
                     uint32_t lo = 0;
                     if (fbeg + lane < fend) {
@@ -148,8 +168,8 @@ int main(int argc, char **argv)
                             lo++;
                         }
                         const uint32_t p = P.run_bin[b.z + lo] + (f - rflat[lo]);
-                        CHECK(p < P.pairs);
-                        bins[p] = contrib[P.psrc[p]];
+                        CHECK(p < P.pairs && f < P.pairs && P.psrc[f] < b.y - b.x);
+                        bins[p] = contrib[P.psrc[f]];
                     }
                 }
             }
