@@ -147,7 +147,8 @@ inline bool build_plan(const uint32_t *off, const uint32_t *cols, uint32_t V, ui
         edgesK[k] = off[r1] - off[r0];
         u += pairsK[k];
         ep += (static_cast<uint64_t>(edgesK[k]) + 7) & ~7ull;
-        if (u >= 0x7FFFFFFFull || ep >= 0xFFFFFFF0ull) {
+        // pairs and codes are addressed with 32-bit byte offsets
+        if (u >= (1ull << 30) || ep >= (1ull << 31) - 8) {
             err = "graph too large for the binned layout";
             return false;
         }

@@ -548,7 +548,9 @@ __global__ __launch_bounds__(kThreads, kRowsLog == 13 ? 8 : 4) void pr_bins_accu
 #pragma unroll
         for (int q = 0; q < kVec; q++) {
             const uint32_t e = min(e0 + lane * kLaneEdges + 8 * q, a.ncodes - 8);
-            raw[q] = __builtin_nontemporal_load(reinterpret_cast<const u32x4 *>(a.codes + e));
+            const uint32_t off = e << 1;  // bytes; codes < 2^31
+            raw[q] = __builtin_nontemporal_load(
+                reinterpret_cast<const u32x4 *>(reinterpret_cast<const char *>(a.codes) + off));
         }
     };
     for (uint32_t c = blockIdx.x; c < a.nchunks; c += gridDim.x - 1) {
@@ -577,13 +579,16 @@ __global__ __launch_bounds__(kThreads, kRowsLog == 13 ? 8 : 4) void pr_bins_accu
             auto load_next = [&](u32x4 (&raw)[kVec]) {
                 load_codes(le, raw);
                 if (lt >= ch.y) return false;
-                le += kRound;
-                if (le >= ltk.y) {
+                if (le + kRound < ltk.y) {
+                    le += kRound;
+                } else {
                     lt = __builtin_amdgcn_readfirstlane(lt + kWaves);
                     if (lt < ch.y) {
                         ltk = stask[lt - ch.x];
                         le = ltk.x;
                     }
+                    // else le stays: the loads past the wave's last round
+                    // repeat its codes, which the caches still hold
                 }
                 return true;
             };
@@ -602,10 +607,14 @@ __global__ __launch_bounds__(kThreads, kRowsLog == 13 ? 8 : 4) void pr_bins_accu
                 for (int q = 0; q < kVec; q++)
 #pragma unroll
                     for (int h = 0; h < 4; h++) cd[4 * q + h] = raw[q][h];
-                uint32_t cnt = 0;
+                // The lane's new-pair flags as a bit mask (edge j = bit j), cut
+                // to its edge count: flags past it are not counted, so a
+                // masked load repeats the lane's last pair (no new line).
+                uint32_t f = 0;
 #pragma unroll
-                for (int j = 0; j < kLaneEdges; j++)
-                    cnt += (j < static_cast<int>(nn) && (code_of(cd, j) & bins::kNewPair)) ? 1u : 0u;
+                for (int j = 0; j < kLaneEdges; j++) f |= ((code_of(cd, j) >> 15) & 1u) << j;
+                f &= (1u << nn) - 1u;  // nn <= 16
+                const uint32_t cnt = static_cast<uint32_t>(__popc(f));
                 uint32_t pre = 0, tot = 0;
 #pragma unroll
                 for (int b = 0; b < kCountBits; b++) {
@@ -613,11 +622,14 @@ __global__ __launch_bounds__(kThreads, kRowsLog == 13 ? 8 : 4) void pr_bins_accu
                     pre += static_cast<uint32_t>(__popcll(m & below)) << b;
                     tot += static_cast<uint32_t>(__popcll(m)) << b;
                 }
-                uint32_t p = base + pre;
+                const uint32_t p0 = base + pre;
 #pragma unroll
                 for (int j = 0; j < kLaneEdges; j++) {
-                    p += (code_of(cd, j) & bins::kNewPair) ? 1u : 0u;
-                    vv[j] = a.bins[min(p, a.npairs - 1)];  // unconditional; masked when added
+                    const uint32_t p = p0 + static_cast<uint32_t>(__popc(f & ((2u << j) - 1u)));
+                    // 32-bit byte offset from the base (pairs < 2^30): the load
+                    // takes the base in SGPRs, no 64-bit address per lane
+                    const uint32_t off = min(p, a.npairs - 1) << 2;
+                    vv[j] = *reinterpret_cast<const float *>(reinterpret_cast<const char *>(a.bins) + off);
                 }
                 base += tot;
                 se += kRound;
