@@ -57,7 +57,7 @@ struct Plan {
     uint32_t V = 0, E = 0, rows_log2 = 0, K = 0;
     uint32_t pairs = 0;                  // U, the number of (source, partition) pairs
     uint32_t slots = 0;                  // partial vectors (chunks)
-    std::vector<uint16_t> codes;         // per edge, partition-major, each partition padded to 8:
+    std::vector<uint16_t> codes;         // per edge, partition-major, each partition padded to 64:
                                          // (row - partition base) | kNewPair on a pair's first edge
     std::vector<uint32_t> edge_start;    // K + 1: padded start of each partition's edges
     std::vector<uint32_t> pair_start;    // K + 1: first pair of each partition
@@ -146,7 +146,7 @@ inline bool build_plan(const uint32_t *off, const uint32_t *cols, uint32_t V, ui
         const uint32_t r0 = k << L, r1 = std::min<uint64_t>(static_cast<uint64_t>(r0) + D, V);
         edgesK[k] = off[r1] - off[r0];
         u += pairsK[k];
-        ep += (static_cast<uint64_t>(edgesK[k]) + 7) & ~7ull;
+        ep += (static_cast<uint64_t>(edgesK[k]) + 63) & ~63ull;  // 64 codes = one 128-byte line
         // pairs and codes are addressed with 32-bit byte offsets
         if (u >= (1ull << 30) || ep >= (1ull << 31) - 8) {
             err = "graph too large for the binned layout";
@@ -248,8 +248,8 @@ inline bool build_plan(const uint32_t *off, const uint32_t *cols, uint32_t V, ui
     }
 
     // Chunks: each partition's edges split evenly (boundaries rounded down
-    // to 8 edges), each chunk's evenly into kChunkWaves tasks (empty ones
-    // dropped).
+    // to 64 edges, so a wave's 512 codes are 8 whole lines), each chunk's
+    // evenly into kChunkWaves tasks (empty ones dropped).
     for (uint32_t k = 0; k < K; k++) {
         const uint64_t n = edgesK[k], nch = P.chunk_base[k + 1] - P.chunk_base[k];
         const uint32_t es = P.edge_start[k];
@@ -258,13 +258,13 @@ inline bool build_plan(const uint32_t *off, const uint32_t *cols, uint32_t V, ui
             for (; counted < off; counted++) flags += (P.codes[es + counted] & kNewPair) ? 1u : 0u;
             return flags;
         };
-        auto round8 = [](uint64_t x) { return static_cast<uint32_t>(x & ~7ull); };
+        auto round64 = [](uint64_t x) { return static_cast<uint32_t>(x & ~63ull); };
         for (uint64_t c = 0; c < nch; c++) {
-            const uint32_t cb = round8(n * c / nch), ce = c + 1 == nch ? static_cast<uint32_t>(n) : round8(n * (c + 1) / nch);
+            const uint32_t cb = round64(n * c / nch), ce = c + 1 == nch ? static_cast<uint32_t>(n) : round64(n * (c + 1) / nch);
             const uint32_t t0 = static_cast<uint32_t>(P.tasks.size());
             for (uint32_t w = 0; w < kChunkWaves; w++) {
-                const uint32_t tb = cb + round8(static_cast<uint64_t>(ce - cb) * w / kChunkWaves);
-                const uint32_t te = w + 1 == kChunkWaves ? ce : cb + round8(static_cast<uint64_t>(ce - cb) * (w + 1) / kChunkWaves);
+                const uint32_t tb = cb + round64(static_cast<uint64_t>(ce - cb) * w / kChunkWaves);
+                const uint32_t te = w + 1 == kChunkWaves ? ce : cb + round64(static_cast<uint64_t>(ce - cb) * (w + 1) / kChunkWaves);
                 if (te > tb) P.tasks.push_back({es + tb, es + te, P.pair_start[k] - 1u + flags_before(tb), k});
             }
             P.chunks.push_back({t0, static_cast<uint32_t>(P.tasks.size()), k, P.chunk_base[k] + static_cast<uint32_t>(c)});

@@ -486,7 +486,6 @@ struct BinArgs {
     const uint4 *tasks;         // (edge begin, edge end, pair base, partition)
     const uint4 *chunks;        // (task begin, task end, partition, slot)
     uint32_t nchunks;
-    uint32_t ncodes;            // length of codes (a multiple of 8, at least 8)
     uint32_t npairs;            // length of bins (at least 1)
     float *bins;                // one contribution per pair
     float *partials;            // [slot][rows per partition]
@@ -540,14 +539,15 @@ __global__ __launch_bounds__(kThreads, kRowsLog == 13 ? 8 : 4) void pr_bins_accu
     const int lane = threadIdx.x & (kWave - 1);
     const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
     const uint64_t below = (1ull << lane) - 1ull;
-    // The lane's codes of the round starting at edge e0.  The loads are
-    // unconditional (no branch, so the compiler can keep several rounds in
-    // flight): a group past the end of the codes is clamped to their last
-    // 8, and codes past the task's end are masked by the edge count later.
-    auto load_codes = [&](uint32_t e0, u32x4 (&raw)[kVec]) {
+    // The lane's codes of the round starting at edge e0 of a task ending at
+    // `end`.  The loads are unconditional (no branch, so the compiler can
+    // keep several rounds in flight): a group past the task's end reads the
+    // task's last group again (a line already fetched) and is masked by the
+    // edge count later.
+    auto load_codes = [&](uint32_t e0, uint32_t end, u32x4 (&raw)[kVec]) {
 #pragma unroll
         for (int q = 0; q < kVec; q++) {
-            const uint32_t e = min(e0 + lane * kLaneEdges + 8 * q, a.ncodes - 8);
+            const uint32_t e = min(e0 + lane * kLaneEdges + 8 * q, (end - 1u) & ~7u);
             const uint32_t off = e << 1;  // bytes; codes < 2^31
             raw[q] = __builtin_nontemporal_load(
                 reinterpret_cast<const u32x4 *>(reinterpret_cast<const char *>(a.codes) + off));
@@ -577,7 +577,7 @@ __global__ __launch_bounds__(kThreads, kRowsLog == 13 ? 8 : 4) void pr_bins_accu
             // at the end), so `raw` never merges an old value with a load in
             // flight.
             auto load_next = [&](u32x4 (&raw)[kVec]) {
-                load_codes(le, raw);
+                load_codes(le, ltk.y, raw);
                 if (lt >= ch.y) return false;
                 if (le + kRound < ltk.y) {
                     le += kRound;
@@ -642,16 +642,17 @@ __global__ __launch_bounds__(kThreads, kRowsLog == 13 ? 8 : 4) void pr_bins_accu
                     }
                 }
             };
-            // Adds a staged round.  Masked edges add +0.0 (to some row of the
-            // partition), which leaves every sum unchanged: contributions are
-            // never negative.  The mask is applied here, a round after the
-            // loads, so that waiting for them never waits for the next
-            // round's.
+            // Adds a staged round.  Masked edges add nothing: their codes
+            // repeat one group (the loads past a task's end are clamped to
+            // its last group), so adding zeros there would pile every masked
+            // lane onto the same few LDS words.  The mask is applied here, a
+            // round after the loads, so that waiting for them never waits for
+            // the next round's.
             auto add = [&](const uint32_t (&cd)[kLaneEdges / 2], const float (&vv)[kLaneEdges], uint32_t nn) {
 #pragma unroll
                 for (int j = 0; j < kLaneEdges; j++)
-                    atomicAdd(&acc[code_of(cd, j) & (kRows - 1)],
-                              j < static_cast<int>(nn) ? static_cast<double>(vv[j]) : 0.0);
+                    if (j < static_cast<int>(nn))
+                        atomicAdd(&acc[code_of(cd, j) & (kRows - 1)], static_cast<double>(vv[j]));
             };
             // kDepth rounds of pair values in flight.  Rounds R_0, R_1, ... of
             // the wave; per step r: the codes of R_{r+kDepth+1} are loaded,
@@ -868,7 +869,7 @@ struct bw_pr_plan {
     const void *roff = nullptr;  // the graph's reverse offsets (for building the bins)
     // Binned mode (bw_pr_plan_bin): 2^bin_log2 rows per partition, 0 = off.
     struct Bins {
-        uint32_t log2 = 0, chunk_tasks = 0, nchunks = 0, nblocks = 0, pairs = 0, slots = 0, K = 0, ncodes = 0;
+        uint32_t log2 = 0, chunk_tasks = 0, nchunks = 0, nblocks = 0, pairs = 0, slots = 0, K = 0;
         uint32_t lane_edges = 8;  // edge codes per lane and round of the accumulate kernel
         uint16_t *codes = nullptr;
         uint4 *tasks = nullptr, *chunks = nullptr, *blocks = nullptr;
@@ -1154,7 +1155,6 @@ extern "C" int bw_pr_plan_bin(bw_pr_plan *plan, int32_t rows_log2, int32_t chunk
     B.nchunks = static_cast<uint32_t>(P.chunks.size());
     B.nblocks = static_cast<uint32_t>(P.blocks.size());
     B.pairs = P.pairs;
-    B.ncodes = static_cast<uint32_t>(P.codes.size());
     B.slots = P.slots;
     B.K = P.K;
     plan->binned = true;
@@ -1211,7 +1211,6 @@ int run_binned(const bw_pr_plan *plan, int32_t iterations, int32_t mode, int32_t
     a.tasks = B.tasks;
     a.chunks = B.chunks;
     a.nchunks = B.nchunks;
-    a.ncodes = B.ncodes;
     a.npairs = std::max<uint32_t>(B.pairs, 1);
     a.bins = B.vals;
     a.partials = B.partials;

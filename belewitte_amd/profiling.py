@@ -38,10 +38,18 @@ def copy(mib: int, repeat: int) -> None:
     src = DeviceArray(n, np.uint32)
     dst = DeviceArray(n, np.uint32)
     src.fill_u32(7)
+    from .device import Event
+    e0, e1 = Event(), Event()
+    best = float("inf")
     for _ in range(repeat):
+        e0.record()
         N.check_hip(N.hip_lib().bw_hip_copy_kernel(dst.ptr, src.ptr, n, None))
+        e1.record()
+        e1.synchronize()
+        best = min(best, e0.elapsed_ms(e1))
     synchronize()
-    print(f"copy: {repeat} x {n * 4} bytes", flush=True)
+    print(f"copy: {repeat} x {n * 4} bytes, best {best:.3f} ms, {2 * n * 4 / best / 1e6:.0f} GB/s "
+          f"(read + write)", flush=True)
 
 
 LINE_FLOATS = 32  # floats per 128-byte cache line
