@@ -12,6 +12,7 @@
 #include <cstring>
 #include <fstream>
 #include <iomanip>
+#include <iostream>
 #include <limits>
 #include <string>
 #include <vector>
@@ -430,7 +431,17 @@ struct FusedPageRank : public ImplementationBase {
         const uint32_t grid = rowsGrid > 0 ? static_cast<uint32_t>(rowsGrid) : 3u * static_cast<uint32_t>(props.compute_units);
         BW_CHK(bw_pr_plan_tune(plan, 0, grid, 1));
         BW_CHK(bw_pr_plan_sort_windows(plan, windows, 16));
-        BW_CHK(bw_pr_plan_bin(plan, binRows, binChunk, 8));
+        // The binned layout holds at most 2048 partitions: larger graphs
+        // take 2^14-row partitions, and beyond that the pull kernels.
+        int rows = binRows;
+        auto partitions = [V](int r) { return (V + (uint64_t{1} << r) - 1) >> r; };
+        if (rows == 13 && partitions(13) > 2048) rows = 14;
+        if (rows != 0 && partitions(rows) > 2048) {
+            std::cerr << "vertex-pull-nodiv-fused: " << V << " vertices need more than 2048 partitions of 2^"
+                      << rows << " rows; using the pull kernels" << std::endl;
+            rows = 0;
+        }
+        BW_CHK(bw_pr_plan_bin(plan, rows, binChunk, 8));
         float *ranks = static_cast<float *>(upload(nullptr, 4 * V));
         float *scratch = static_cast<float *>(upload(nullptr, 4 * V));
         double *acc = static_cast<double *>(upload(nullptr, 8 * bw_pr_fused_acc_len(max_iterations)));

@@ -172,12 +172,16 @@ class FusedPageRank:
                                ("bins", 14, 32, 8), (0, 3 * cus, 1, 2, 16), (0, 2 * cus, 1, 4, 16)]
         table = {}
         for shape in cands:
-            self._apply(shape)
+            try:
+                self._apply(shape)
+            except N.BelewitteError:
+                continue  # a shape this graph cannot take (too many partitions)
             self.run(iterations, "uniform", download=False)
             table[shape] = min(self.run(iterations, "uniform", download=False)[2] for _ in range(3))
         best = min(table, key=table.get)
-        if table[best] > (1.0 - margin) * table[cands[0]]:
-            best = cands[0]
+        first = next(s for s in cands if s in table)
+        if table[best] > (1.0 - margin) * table[first]:
+            best = first
         self._apply(best)
         return {self._shape_name(s): round(v, 4) for s, v in table.items()}
 
@@ -199,7 +203,7 @@ class FusedPageRank:
         """The kernels one iteration launches under the current shape."""
         if self.binning:
             r, _, lanes = self.binning
-            return f"pr_bins_accumulate_kernel<{r}, 1024, {lanes}> + pr_bins_finalise_kernel<{r}, 512>"
+            return f"pr_bins_accumulate_kernel<{r}, 1024, {lanes}, 2> + pr_bins_finalise_kernel<{r}, 512>"
         hot, _, tiles, width, batch = self.tuning
         rows = f"pr_windows_kernel<{width}, {batch}>" if width else f"pr_rows_kernel<{hot}, {tiles}>"
         return f"pr_pieces_kernel + {rows}"

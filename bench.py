@@ -56,7 +56,11 @@ def traffic_from_profile(fused):
         rec = json.loads(p.read_text())
     except (OSError, ValueError):
         return None
-    if fused.binning and tuple(rec.get("binning", ())) != tuple(fused.binning):
+    if fused.binning:
+        # one record per binned shape the autotuner may choose
+        for r in rec.get("shapes", []):
+            if tuple(r.get("binning", ())) == tuple(fused.binning):
+                return r.get("bytes_per_launch")
         return None
     return rec.get("bytes_per_launch")
 
