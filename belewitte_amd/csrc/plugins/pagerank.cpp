@@ -354,7 +354,7 @@ struct FusedPageRank : public ImplementationBase {
     int windows = 2;
     int rowsGrid = 0;
     int binRows = 13;
-    int binChunk = 32;
+    int binChunk = 64;
     bw_graph *host = nullptr;
     bw_graph_dev dev{};
     std::vector<uint32_t> perm;  // file id -> device id, when relabelled
