@@ -121,3 +121,20 @@ def test_binned_rejects_bad_settings(gpu):
             fp.bin(*args)
     fp.close()
     dg.free()
+
+
+@pytest.mark.parametrize("shape", [(13, 64, 8), (14, 128, 8), (13, 32, 16)])
+def test_binned_agrees_with_pull_at_scale_20(gpu, shape):
+    """R-MAT scale 20, edge factor 16, degree-ordered: the binned iteration
+    against the Kahan-compensated pull kernel (same recurrence) to FP32
+    rounding of the ranks, and UNIFORM mode conserving the rank mass."""
+    g = graph.rmat(20, 16, seed=7)
+    fp = pagerank.FusedPageRank.for_graph(g, max_iterations=30, reorder=True)
+    fp.tune(0, 0, 1, 2, 16)
+    pull, rp, _ = fp.run(30, "uniform")
+    fp.bin(*shape)
+    ranks, resid, _ = fp.run(30, "uniform")
+    assert float(np.max(np.abs(ranks - pull))) <= 4e-7 * float(pull.max()), shape
+    assert abs(float(np.sum(ranks, dtype=np.float64)) - 1.0) <= 1e-5
+    assert np.allclose(resid, rp, rtol=1e-3, atol=1e-6)
+    fp.close()
