@@ -561,10 +561,9 @@ __global__ __launch_bounds__(kThreads, kRowsLog == 13 ? 8 : 4) void pr_bins_accu
         // make every wait for it a wait for all the loads in flight.
         for (uint32_t i = threadIdx.x; i < ch.y - ch.x; i += kThreads) stask[i] = a.tasks[ch.x + i];
         __syncthreads();
-        // The wave's rounds, over all its tasks of the chunk (t = ch.x + wid,
-        // t + kWaves, ...), are software-pipelined: the codes of round r + 2
-        // and the pair values of round r + 1 are in flight while round r
-        // adds, also across task boundaries.
+        // The wave's rounds, over its tasks of the chunk (t = ch.x + wid,
+        // t + kWaves, ...: one task per wave with the plan's even splits),
+        // are software-pipelined as set out above the loop below.
         uint32_t t = __builtin_amdgcn_readfirstlane(ch.x + wid);
         if (t < ch.y) {
             uint4 tk = stask[t - ch.x];   // task of the round being staged
