@@ -1149,8 +1149,8 @@ extern "C" int bw_pr_plan_bin(bw_pr_plan *plan, int32_t rows_log2, int32_t chunk
         plan->binned = true;
         return BW_HIP_OK;
     }
-    B.release();
-    plan->binned = false;
+    // A new shape is built beside the current one and replaces it only once
+    // it is complete: a failed call leaves the plan as it was.
     const uint32_t V = plan->V, E = plan->E;
     std::vector<uint32_t> off(static_cast<size_t>(V) + 1), cols(std::max<uint32_t>(E, 1));
     BW_HIP_CHECK(hipMemcpy(off.data(), plan->roff, sizeof(uint32_t) * (static_cast<size_t>(V) + 1),
@@ -1173,30 +1173,33 @@ extern "C" int bw_pr_plan_bin(bw_pr_plan *plan, int32_t rows_log2, int32_t chunk
         if (!vec.empty() && hipMemcpy(dst, vec.data(), sizeof(T) * vec.size(), hipMemcpyHostToDevice) != hipSuccess)
             ok = false;
     };
-    put(B.codes, P.codes);
-    put(B.tasks, P.tasks);
-    put(B.chunks, P.chunks);
-    put(B.blocks, P.blocks);
-    put(B.chunk_base, P.chunk_base);
-    put(B.run_flat, P.run_flat);
-    put(B.run_bin, P.run_bin);
-    put(B.psrc, P.psrc);
+    bw_pr_plan::Bins nb;
+    put(nb.codes, P.codes);
+    put(nb.tasks, P.tasks);
+    put(nb.chunks, P.chunks);
+    put(nb.blocks, P.blocks);
+    put(nb.chunk_base, P.chunk_base);
+    put(nb.run_flat, P.run_flat);
+    put(nb.run_bin, P.run_bin);
+    put(nb.psrc, P.psrc);
     const size_t rows = size_t{1} << rows_log2;
     // 8 values of padding: the accumulate reads 8-pair windows from a lane's first pair
-    if (!ok || hipMalloc(&B.vals, sizeof(float) * (static_cast<size_t>(P.pairs) + 8)) != hipSuccess ||
-        hipMalloc(&B.partials, sizeof(float) * rows * std::max<size_t>(P.slots, 1)) != hipSuccess ||
-        hipMalloc(&B.part, sizeof(double2) * std::max<size_t>(P.blocks.size(), 1)) != hipSuccess) {
-        B.release();
-        return fail(BW_HIP_ERR_NOMEM, "device allocation of the binned PageRank plan failed");
+    if (!ok || hipMalloc(&nb.vals, sizeof(float) * (static_cast<size_t>(P.pairs) + 8)) != hipSuccess ||
+        hipMalloc(&nb.partials, sizeof(float) * rows * std::max<size_t>(P.slots, 1)) != hipSuccess ||
+        hipMalloc(&nb.part, sizeof(double2) * std::max<size_t>(P.blocks.size(), 1)) != hipSuccess) {
+        nb.release();
+        return fail(BW_HIP_ERR_NOMEM, "device allocation or upload of the binned PageRank plan failed");
     }
-    B.log2 = static_cast<uint32_t>(rows_log2);
-    B.chunk_tasks = static_cast<uint32_t>(chunk_tasks);
-    B.lane_edges = static_cast<uint32_t>(lane_edges);
-    B.nchunks = static_cast<uint32_t>(P.chunks.size());
-    B.nblocks = static_cast<uint32_t>(P.blocks.size());
-    B.pairs = P.pairs;
-    B.slots = P.slots;
-    B.K = P.K;
+    nb.log2 = static_cast<uint32_t>(rows_log2);
+    nb.chunk_tasks = static_cast<uint32_t>(chunk_tasks);
+    nb.lane_edges = static_cast<uint32_t>(lane_edges);
+    nb.nchunks = static_cast<uint32_t>(P.chunks.size());
+    nb.nblocks = static_cast<uint32_t>(P.blocks.size());
+    nb.pairs = P.pairs;
+    nb.slots = P.slots;
+    nb.K = P.K;
+    B.release();
+    B = nb;
     plan->binned = true;
     return BW_HIP_OK;
 }

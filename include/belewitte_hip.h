@@ -427,7 +427,8 @@ int bw_pr_plan_sort_windows(bw_pr_plan *plan, int32_t tiles_per_window,
  * (pagerank/vertex_pull_nodiv.cu:3-29, pagerank/pagerank.cu:142-165); the
  * LDS additions run in a run-dependent order, but each partial is an FP64
  * sum rounded once to FP32.  Needs V / 2^rows_log2 <= 2048.  rows_log2 0
- * returns to the pull kernels. */
+ * returns to the pull kernels.  On error the plan keeps the mode and shape
+ * it had before the call. */
 int bw_pr_plan_bin(bw_pr_plan *plan, int32_t rows_log2, int32_t chunk_tasks,
                    int32_t lane_edges);
 /* Pairs, chunks and scatter blocks of the binned plan (0 when off). */

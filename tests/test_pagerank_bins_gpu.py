@@ -123,6 +123,27 @@ def test_binned_rejects_bad_settings(gpu):
     dg.free()
 
 
+def test_binned_failed_shape_keeps_plan(gpu):
+    """A chain of 2^24 + 2 vertices: 1025 partitions at 2^14 rows, 2049 at
+    2^13, one more than the layout holds.  The refused 2^13 shape leaves
+    the plan binned at 2^14, and its ranks still match the oracle."""
+    g = graph.chain((1 << 24) + 1)
+    csr = O.CSR.from_graph(g)
+    dg = device.DeviceGraph(g)
+    fp = pagerank.FusedPageRank(dg, max_iterations=5)
+    fp.bin(14, 64, 8)
+    before = fp.bin_info()
+    with pytest.raises(Exception, match="too many partitions"):
+        fp.bin(13, 64, 8)
+    assert fp.binning == (14, 64, 8)
+    assert fp.bin_info() == before and before["pairs"] > 0
+    ranks, resid, _ = fp.run(5, "uniform")
+    assert float(np.max(np.abs(ranks - O.pagerank_f64(csr, 5, "uniform")))) <= TOL
+    assert np.all(np.isfinite(resid))
+    fp.close()
+    dg.free()
+
+
 @pytest.mark.parametrize("shape", [(13, 64, 8), (14, 128, 8), (13, 32, 16)])
 def test_binned_agrees_with_pull_at_scale_20(gpu, shape):
     """R-MAT scale 20, edge factor 16, degree-ordered: the binned iteration
