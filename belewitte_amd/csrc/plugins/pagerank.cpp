@@ -353,8 +353,8 @@ struct FusedPageRank : public ImplementationBase {
     bool keepOrder = false;
     int windows = 2;
     int rowsGrid = 0;
-    int binRows = 13;
-    int binChunk = 64;
+    int binRows = 14;
+    int binChunk = 128;
     bw_graph *host = nullptr;
     bw_graph_dev dev{};
     std::vector<uint32_t> perm;  // file id -> device id, when relabelled

@@ -182,7 +182,7 @@ def test_fused_pagerank_through_the_runner(gfile, tmp_path):
     cases = [([], 100, "drop"), (["--dangling", "uniform", "--iterations", "30"], 30, "uniform"),
              (["--no-reorder", "--bins", "0", "--windows", "0"], 100, "drop"),
              (["--bins", "0", "--windows", "4", "--rows-grid", "7"], 100, "drop"),
-             (["--bins", "14", "--bin-chunk", "1"], 100, "drop")]
+             (["--bins", "13", "--bin-chunk", "1"], 100, "drop")]
     for extra, iters, mode in cases:
         run(["-a", "pagerank", "-k", "vertex-pull-nodiv-fused", "--validate", *extra, "-o", str(tmp_path), str(p)])
         vals = np.array([float(l.split("\t")[1]) for l in (tmp_path / "rmat11.output").read_text().splitlines()])
