@@ -671,12 +671,30 @@ __global__ __launch_bounds__(kThreads, kRowsLog == 13 ? 8 : 4) void pr_bins_accu
             // the next round's.
             auto add = [&](const uint32_t (&cd)[kLaneEdges / 2], const float (&vv)[kLaneEdges], uint32_t nn,
                            uint32_t fl) {
+                // Window: edge j's pair is window entry o_j, the number of
+                // new-pair flags of edges 1..j (0 <= o_j <= j).  With 2^14 rows
+                // (4 waves per SIMD, registers to spare), w is the window
+                // shifted left by o_j: each flag shifts it by one, entry 0 is
+                // the edge's value, and edge j shifts entries 0 .. 7 - j only,
+                // since edge j + m reads no entry past m.  That takes half the
+                // VALU instructions of comparing o_j with each candidate, but
+                // more registers than the 2^13-row kernel's 64 (8 waves per
+                // SIMD), which keeps the comparisons.
+                constexpr bool kShift = kWindow && kRowsLog == 14;
+                float w[kLaneEdges];
+#pragma unroll
+                for (int i = 0; i < kLaneEdges; i++) w[i] = vv[i];
 #pragma unroll
                 for (int j = 0; j < kLaneEdges; j++) {
                     float x = vv[j];
-                    if constexpr (kWindow) {
-                        // edge j's pair is window entry o (0 <= o <= j): the
-                        // new-pair flags of edges 1..j
+                    if constexpr (kShift) {
+                        if (j > 0) {
+                            const bool s = (fl >> j) & 1u;
+#pragma unroll
+                            for (int i = 0; i < kLaneEdges - j; i++) w[i] = s ? w[i + 1] : w[i];
+                        }
+                        x = w[0];
+                    } else if constexpr (kWindow) {
                         const uint32_t o = static_cast<uint32_t>(__popc((fl >> 1) & ((1u << j) - 1u)));
                         x = vv[0];
 #pragma unroll
