@@ -143,7 +143,7 @@ class FusedPageRank:
         N.check_hip(lib.bw_pr_plan_sort_windows(self._plan, int(sorted_windows), int(batch)))
         self.tuning = (int(hot), int(rows_grid), int(tiles), int(sorted_windows), int(batch))
 
-    def bin(self, rows_log2: int = 13, chunk_tasks: int = 64, lane_edges: int = 8) -> None:
+    def bin(self, rows_log2: int = 14, chunk_tasks: int = 128, lane_edges: int = 8) -> None:
         """Switches to the binned iteration (propagation blocking): rows cut
         into partitions of 2**rows_log2 (13, 14) accumulated in LDS in FP64,
         one contribution per distinct (source, partition) pair written and
