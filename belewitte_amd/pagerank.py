@@ -89,7 +89,9 @@ FUSED = "vertex-pull-nodiv-fused"
 
 
 class FusedPageRank:
-    """The MI355X-native fused pull iteration (one launch per iteration).
+    """The MI355X-native fused iteration of vertex-pull-nodiv: the pull
+    kernel (one launch per iteration) or the binned iteration (an
+    accumulate and a finalise launch per iteration, see bin()).
 
     Holds the plan and the workspaces, so repeated runs allocate nothing."""
 
