@@ -120,8 +120,6 @@ _HIP_SIGS = {
     "bw_hip_event_sync": (C.c_int, [vp]),
     "bw_hip_event_elapsed_ms": (C.c_int, [f32p, vp, vp]),
     "bw_hip_compute_division": (C.c_int, [C.c_uint64, u64p, u64p]),
-    "bw_hip_copy_kernel": (C.c_int, [vp, vp, C.c_uint64, vp]),
-    "bw_hip_gather_probe": (C.c_int, [vp, C.c_uint64, vp, C.c_uint32, vp, C.c_uint32, C.POINTER(C.c_float), vp]),
     # BFS
     "bw_bfs_edge_list": (C.c_int, [C.c_uint64, vp, vp, vp, C.c_int32, vp, C.c_int32, _L]),
     "bw_bfs_rev_edge_list": (C.c_int, [C.c_uint64, vp, vp, vp, C.c_int32, vp, C.c_int32, _L]),

@@ -6,6 +6,7 @@
 Outputs (git-ignored, shipped with the tree):
     belewitte_amd/lib/libbwgraph.so          host graph substrate (g++ + OpenMP)
     belewitte_amd/lib/libbelewitte_hip.so    HIP kernels + C ABI (hipcc, gfx950)
+    belewitte_amd/lib/libbw_probes.so        measurement probes (profiling only)
     belewitte_amd/lib/kernels/lib{bfs,pagerank}kernel.so   plugin libraries
     belewitte_amd/bin/kernel-runner          drop-in runner of the plugins
 """
@@ -87,6 +88,15 @@ def build_hip() -> Path:
     return out
 
 
+def build_probes() -> Path:
+    """The measurement probes of belewitte_amd.profiling (copy kernel for
+    counter calibration, gather-rate probe); not part of the kernel ABI."""
+    LIB.mkdir(parents=True, exist_ok=True)
+    out = LIB / "libbw_probes.so"
+    _run([HIPCC, *HIP_FLAGS, "-shared", str(CSRC / "profiling" / "probes.hip"), "-o", str(out)])
+    return out
+
+
 def build_oracle() -> Path:
     _run(["make", "-s", "-C", str(ROOT / "oracle")])
     return ROOT / "oracle" / "_build" / "liboracle.so"
@@ -125,7 +135,8 @@ def build_framework() -> list:
 
 
 def build_all(verbose: bool = True) -> None:
-    steps = [("graph", build_graph), ("hip", build_hip), ("oracle", build_oracle), ("framework", build_framework)]
+    steps = [("graph", build_graph), ("hip", build_hip), ("probes", build_probes), ("oracle", build_oracle),
+             ("framework", build_framework)]
     for name, fn in steps:
         r = fn()
         if verbose:
@@ -134,8 +145,8 @@ def build_all(verbose: bool = True) -> None:
 
 def main(argv) -> int:
     targets = argv[1:] or ["all"]
-    table = {"graph": build_graph, "hip": build_hip, "oracle": build_oracle, "framework": build_framework,
-             "all": build_all}
+    table = {"graph": build_graph, "hip": build_hip, "probes": build_probes, "oracle": build_oracle,
+             "framework": build_framework, "all": build_all}
     for t in targets:
         print(f"[build] {t}: {table[t]()}", flush=True)
     return 0

@@ -666,6 +666,26 @@ extern "C" int bw_graph_degree_stats(const bw_graph *g, int kind, double out[7])
     });
 }
 
+// Relabels one stored CSR: new row i is old row order[i], its columns
+// mapped through perm and sorted.  A pure permutation: every stored edge,
+// duplicates and self-loops included, keeps its multiplicity.
+static void permute_csr(uint64_t V, const std::vector<uint64_t> &off, const std::vector<uint32_t> &col,
+                        const std::vector<uint32_t> &order, const std::vector<uint32_t> &perm,
+                        std::vector<uint64_t> &noff, std::vector<uint32_t> &ncol)
+{
+    noff.assign(V + 1, 0);
+    for (uint64_t i = 0; i < V; i++) noff[i + 1] = noff[i] + (off[order[i] + 1] - off[order[i]]);
+    ncol.resize(noff[V]);
+#pragma omp parallel for schedule(dynamic, 4096)
+    for (int64_t i = 0; i < static_cast<int64_t>(V); i++) {
+        const uint64_t v = order[static_cast<uint64_t>(i)];
+        uint64_t k = noff[static_cast<uint64_t>(i)];
+        for (uint64_t j = off[v]; j < off[v + 1]; j++) ncol[k++] = perm[col[j]];
+        std::sort(ncol.begin() + static_cast<int64_t>(noff[static_cast<uint64_t>(i)]),
+                  ncol.begin() + static_cast<int64_t>(k));
+    }
+}
+
 extern "C" int bw_graph_reorder_by_degree(bw_graph **out, const bw_graph *g,
                                           uint32_t *perm_out)
 {
@@ -680,21 +700,14 @@ extern "C" int bw_graph_reorder_by_degree(bw_graph **out, const bw_graph *g,
         });
         std::vector<uint32_t> perm(V);
         for (uint64_t i = 0; i < V; i++) perm[order[i]] = static_cast<uint32_t>(i);
-        std::vector<uint32_t> s(g->E), d(g->E);
-        for (uint64_t v = 0; v < V; v++) {
-            for (uint64_t j = g->off[v]; j < g->off[v + 1]; j++) {
-                s[j] = perm[v];
-                d[j] = perm[g->col[j]];
-            }
-        }
-        // The edge set is already symmetric for an undirected graph, so it
-        // is rebuilt as a directed CSR and flagged undirected afterwards.
-        auto r = finish(V, s, d, false, true);
-        if (g->undirected) {
-            r->undirected = true;
-            std::vector<uint64_t>().swap(r->roff);
-            std::vector<uint32_t>().swap(r->rcol);
-        }
+        // Both stored directions are permuted as they are (an undirected
+        // graph stores one CSR), so the edge count never changes.
+        auto r = std::make_unique<bw_graph>();
+        r->V = V;
+        r->E = g->E;
+        r->undirected = g->undirected;
+        permute_csr(V, g->off, g->col, order, perm, r->off, r->col);
+        if (!g->undirected) permute_csr(V, g->roff, g->rcol, order, perm, r->roff, r->rcol);
         if (perm_out) std::copy(perm.begin(), perm.end(), perm_out);
         *out = r.release();
         return BW_OK;

@@ -84,50 +84,6 @@ __global__ void fill_u32_kernel(uint32_t *dst, uint32_t value, uint64_t n)
         dst[i] = value;
 }
 
-// One 16-byte element per thread, streamed (non-temporal): the launch
-// covers the array with a thread each, which keeps the most loads in flight
-// (about 6.5 TB/s copied on MI355X, against 4.6 for a grid-stride loop).
-typedef uint32_t copy_u32x4 __attribute__((ext_vector_type(4)));
-__global__ __launch_bounds__(256) void copy_u32x4_kernel(copy_u32x4 *__restrict__ dst,
-                                                         const copy_u32x4 *__restrict__ src, uint64_t n4)
-{
-    const uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i < n4) __builtin_nontemporal_store(__builtin_nontemporal_load(&src[i]), &dst[i]);
-}
-
-__global__ void copy_u32_tail_kernel(uint32_t *dst, const uint32_t *src, uint64_t begin, uint64_t n)
-{
-    uint64_t i = begin + static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i < n) dst[i] = src[i];
-}
-
-// Gather-rate probe: thread t sums table[idx[r * threads + t]] over its
-// rounds, kGatherBatch loads in flight at a time, so consecutive lanes of a
-// wave read consecutive index entries (the access shape of a row tile's
-// gathers).  Indices at or beyond `len` read nothing.
-constexpr int kGatherBatch = 16;
-
-__global__ __launch_bounds__(256) void gather_probe_kernel(const uint32_t *__restrict__ idx,
-                                                           const float *__restrict__ table, uint32_t len,
-                                                           float *__restrict__ out, uint32_t rounds)
-{
-    const uint64_t threads = static_cast<uint64_t>(gridDim.x) * blockDim.x;
-    const uint64_t t = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    float acc = 0.0f;
-    for (uint32_t r = 0; r < rounds; r++) {
-        uint32_t c[kGatherBatch];
-#pragma unroll
-        for (int k = 0; k < kGatherBatch; k++)
-            c[k] = __builtin_nontemporal_load(&idx[(static_cast<uint64_t>(r) * kGatherBatch + k) * threads + t]);
-        float v[kGatherBatch];
-#pragma unroll
-        for (int k = 0; k < kGatherBatch; k++) v[k] = c[k] < len ? table[c[k]] : 0.0f;
-#pragma unroll
-        for (int k = 0; k < kGatherBatch; k++) acc += v[k];
-    }
-    out[t] = acc;
-}
-
 }  // namespace bw
 
 using namespace bw;
@@ -254,56 +210,6 @@ extern "C" int bw_hip_fill_u32(void *dst, uint32_t value, uint64_t count, void *
     hipLaunchKernelGGL(fill_u32_kernel, dim3(grid), dim3(block), 0,
                        static_cast<hipStream_t>(stream), static_cast<uint32_t *>(dst), value, count);
     BW_LAUNCH_CHECK();
-    return BW_HIP_OK;
-}
-
-extern "C" int bw_hip_copy_kernel(uint32_t *dst, const uint32_t *src, uint64_t n, void *stream)
-{
-    if (n == 0) return BW_HIP_OK;
-    if (!dst || !src) return fail(BW_HIP_ERR_ARG, "NULL pointer in copy_kernel");
-    if ((reinterpret_cast<uintptr_t>(dst) | reinterpret_cast<uintptr_t>(src)) & 15)
-        return fail(BW_HIP_ERR_ARG, "copy_kernel needs 16-byte aligned buffers");
-    hipStream_t s = static_cast<hipStream_t>(stream);
-    const uint64_t n4 = n / 4;
-    if (n4) {
-        const uint64_t grid = (n4 + 255) / 256;
-        if (grid > 0x7FFFFFFFull) return fail(BW_HIP_ERR_ARG, "copy_kernel: more than 2^31 workgroups");
-        hipLaunchKernelGGL(copy_u32x4_kernel, dim3(static_cast<uint32_t>(grid)), dim3(256), 0, s,
-                           reinterpret_cast<copy_u32x4 *>(dst), reinterpret_cast<const copy_u32x4 *>(src), n4);
-        BW_LAUNCH_CHECK();
-    }
-    if (n4 * 4 < n) {
-        hipLaunchKernelGGL(copy_u32_tail_kernel, dim3(1), dim3(64), 0, s, dst, src, n4 * 4, n);
-        BW_LAUNCH_CHECK();
-    }
-    return BW_HIP_OK;
-}
-
-extern "C" int bw_hip_gather_probe(const uint32_t *idx, uint64_t n, const float *table, uint32_t table_len,
-                                   float *out, uint32_t threads, float *kernel_ms, void *stream)
-{
-    if (!idx || !table || !out || table_len == 0) return fail(BW_HIP_ERR_ARG, "gather_probe: invalid argument");
-    if (threads == 0 || threads % 256 != 0) return fail(BW_HIP_ERR_ARG, "gather_probe: threads must be a multiple of 256");
-    const uint64_t per_round = static_cast<uint64_t>(threads) * kGatherBatch;
-    if (n == 0 || n % per_round != 0 || n / per_round > 0xFFFFFFFFull)
-        return fail(BW_HIP_ERR_ARG, "gather_probe: n must be a positive multiple of 16 * threads");
-    hipStream_t s = static_cast<hipStream_t>(stream);
-    hipEvent_t e0 = nullptr, e1 = nullptr;
-    if (kernel_ms) {
-        BW_HIP_CHECK(hipEventCreate(&e0));
-        BW_HIP_CHECK(hipEventCreate(&e1));
-        BW_HIP_CHECK(hipEventRecord(e0, s));
-    }
-    hipLaunchKernelGGL(gather_probe_kernel, dim3(threads / 256), dim3(256), 0, s, idx, table, table_len, out,
-                       static_cast<uint32_t>(n / per_round));
-    BW_LAUNCH_CHECK();
-    if (kernel_ms) {
-        BW_HIP_CHECK(hipEventRecord(e1, s));
-        BW_HIP_CHECK(hipEventSynchronize(e1));
-        BW_HIP_CHECK(hipEventElapsedTime(kernel_ms, e0, e1));
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
-    }
     return BW_HIP_OK;
 }
 

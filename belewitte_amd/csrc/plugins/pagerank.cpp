@@ -344,8 +344,14 @@ struct PageRank : public ImplementationTemplate<Platform, Vertex, Edge, switchin
 // --dangling flags and output formats as the implementations above.  By
 // default the vertices are relabelled by decreasing out-degree on upload
 // (--no-reorder keeps the file's order; the output is in the file's order
-// either way) and the row tiles run in source-sorted windows of 2 tiles
-// (--windows 0/1/2/4) on 3 persistent workgroups per CU (--rows-grid N).
+// either way) and each iteration is the binned form (propagation blocking):
+// partitions of 2^14 rows (--bins 13/14; --bins 0 selects the pull
+// kernels), chunks of 128 tasks of 2048 edges (--bin-chunk N), 8 edges per
+// lane.  A graph the binned layout cannot hold (more than 2048 partitions,
+// or too many pairs or edge codes) falls back to the pull kernels with a
+// note on stderr.  --windows 0/1/2/4 (source-sorted windows of that many
+// row tiles) and --rows-grid N (persistent row workgroups, default 3 per
+// CU) shape the pull kernels only.
 // Timer "computation": device time of the iterations.
 struct FusedPageRank : public ImplementationBase {
     int max_iterations = 100;
@@ -388,7 +394,7 @@ struct FusedPageRank : public ImplementationBase {
     {
         freeGraph();
         if (bw_graph_load(&host, file.c_str()) != 0) reportError("Failed to load '", file, "': ", bw_graph_last_error());
-        const uint64_t V = bw_graph_vertex_count(host), E = bw_graph_edge_count(host);
+        const uint64_t V = bw_graph_vertex_count(host);
         const bw_graph *g = host;
         bw_graph *relabelled = nullptr;
         if (!keepOrder) {
@@ -398,6 +404,9 @@ struct FusedPageRank : public ImplementationBase {
         } else {
             perm.clear();
         }
+        // Sizes come from the graph that is uploaded (the relabelling keeps
+        // every stored edge, so its count equals the file's).
+        const uint64_t E = bw_graph_edge_count(g);
         std::vector<uint32_t> off(V + 1);
         if (bw_graph_offsets32(g, 1, off.data()) != 0) reportError(bw_graph_last_error());
         dev.vertex_count = V;
@@ -441,7 +450,15 @@ struct FusedPageRank : public ImplementationBase {
                       << rows << " rows; using the pull kernels" << std::endl;
             rows = 0;
         }
-        BW_CHK(bw_pr_plan_bin(plan, rows, binChunk, 8));
+        int status = bw_pr_plan_bin(plan, rows, binChunk, 8);
+        if (rows != 0 && status == BW_HIP_ERR_ARG) {
+            // Refused on the host (too many pairs or edge codes for the
+            // layout); the plan is unchanged, so the pull kernels run.
+            std::cerr << "vertex-pull-nodiv-fused: the binned layout refused the graph (" << bw_hip_last_error()
+                      << "); using the pull kernels" << std::endl;
+            status = bw_pr_plan_bin(plan, 0, binChunk, 8);
+        }
+        BW_CHK(status);
         float *ranks = static_cast<float *>(upload(nullptr, 4 * V));
         float *scratch = static_cast<float *>(upload(nullptr, 4 * V));
         double *acc = static_cast<double *>(upload(nullptr, 8 * bw_pr_fused_acc_len(max_iterations)));

@@ -1,11 +1,11 @@
 """Small GPU workloads for counter calibration and kernel profiling.
 
     python -m belewitte_amd.profiling copy --mib 1024
-        One streaming copy of the given size (bw_hip_copy_kernel): reads and
+        One streaming copy of the given size (bw_probe_copy): reads and
         writes exactly that many bytes, the reference point that scales the
         FETCH_SIZE / WRITE_SIZE counters to bytes.
     python -m belewitte_amd.profiling gather --table-mib 16 --share 1,2,4,8,16,32,64
-        The gather-rate probe (bw_hip_gather_probe): 64 Mi gathers from a
+        The gather-rate probe (bw_probe_gather): 64 Mi gathers from a
         float table where `share` consecutive lanes read one 128-byte line;
         prints lanes/s and distinct lines/s per sharing factor.
     python -m belewitte_amd.profiling pagerank --scale 22 --iterations 5 [--reorder] [--grid N --hot H --tiles T] [--sorted W --batch B] [--bins R,C,L]
@@ -20,17 +20,48 @@
         workload (the shipped model for the learned policy).
 
 Run them under `rocprofv3 --pmc ... --kernel-trace -- python3 -m ...`.
+The copy and gather probes live in libbw_probes.so, a profiling-only
+library beside the kernel library (csrc/profiling/bw_probes.h).
 """
 from __future__ import annotations
 
 import argparse
+import ctypes as C
+import os
 import sys
 
 import numpy as np
 
+_probes = None
+
+
+def probes_lib() -> C.CDLL:
+    """libbw_probes.so with its signatures (fails loudly when not built)."""
+    global _probes
+    if _probes is None:
+        from . import _native as N
+        path = N.LIB_DIR / "libbw_probes.so"
+        if not path.exists():
+            raise N.NativeLibraryMissing(f"{path} is not built; run belewitte_amd.build.build_all()")
+        lib = C.CDLL(str(path), mode=os.RTLD_NOW)
+        lib.bw_probe_last_error.restype = C.c_char_p
+        lib.bw_probe_last_error.argtypes = []
+        lib.bw_probe_copy.restype = C.c_int
+        lib.bw_probe_copy.argtypes = [C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p]
+        lib.bw_probe_gather.restype = C.c_int
+        lib.bw_probe_gather.argtypes = [C.c_void_p, C.c_uint64, C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32,
+                                        C.POINTER(C.c_float), C.c_void_p]
+        _probes = lib
+    return _probes
+
+
+def check_probe(status: int) -> None:
+    if status != 0:
+        from . import _native as N
+        raise N.BelewitteError(status, probes_lib().bw_probe_last_error().decode())
+
 
 def copy(mib: int, repeat: int) -> None:
-    from . import _native as N
     from .device import DeviceArray, require_gpu, set_device, synchronize
     require_gpu()
     set_device(0)
@@ -43,7 +74,7 @@ def copy(mib: int, repeat: int) -> None:
     best = float("inf")
     for _ in range(repeat):
         e0.record()
-        N.check_hip(N.hip_lib().bw_hip_copy_kernel(dst.ptr, src.ptr, n, None))
+        check_probe(probes_lib().bw_probe_copy(dst.ptr, src.ptr, n, None))
         e1.record()
         e1.synchronize()
         best = min(best, e0.elapsed_ms(e1))
@@ -70,9 +101,6 @@ def gather_indices(n: int, table_len: int, share: int, seed: int = 0, stride: in
 
 def gather(table_mib: int, shares, mi: int = 64, threads: int = 256 * 8192, repeat: int = 5,
            stride: int = 1) -> dict:
-    import ctypes as C
-
-    from . import _native as N
     from .device import DeviceArray, require_gpu, set_device
     require_gpu()
     set_device(0)
@@ -91,8 +119,8 @@ def gather(table_mib: int, shares, mi: int = 64, threads: int = 256 * 8192, repe
         best = float("inf")
         for _ in range(repeat):
             ms = C.c_float()
-            N.check_hip(N.hip_lib().bw_hip_gather_probe(idx.ptr, n, table.ptr, tlen, out.ptr, threads,
-                                                        C.byref(ms), None))
+            check_probe(probes_lib().bw_probe_gather(idx.ptr, n, table.ptr, tlen, out.ptr, threads,
+                                                     C.byref(ms), None))
             best = min(best, ms.value)
         lanes = n / best / 1e6
         res[share] = {"ms": round(best, 4), "G_lanes_per_s": round(lanes, 1),

@@ -62,6 +62,9 @@ def traffic_from_profile(fused):
             if tuple(r.get("binning", ())) == tuple(fused.binning):
                 return r.get("bytes_per_launch")
         return None
+    # The pull record holds one launch shape; another shape's traffic differs.
+    if tuple(rec.get("tuning", ())) != tuple(fused.tuning):
+        return None
     return rec.get("bytes_per_launch")
 
 

@@ -117,7 +117,10 @@ uint32_t bw_graph_max_out_degree_vertex(const bw_graph *g);
 int bw_graph_degree_stats(const bw_graph *g, int kind, double out[7]);
 
 /* Relabels vertices by decreasing out-degree (ties by id), the experiment
- * of reorder-graph.cpp:40-128.  perm_out[old] = new id (optional). */
+ * of reorder-graph.cpp:40-128.  perm_out[old] = new id (optional).  A pure
+ * permutation of each stored CSR: duplicate edges and self-loops keep
+ * their multiplicity, so the edge count never changes; rows come out
+ * sorted. */
 int bw_graph_reorder_by_degree(bw_graph **out, const bw_graph *g,
                                uint32_t *perm_out);
 

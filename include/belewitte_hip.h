@@ -105,22 +105,6 @@ int bw_hip_event_elapsed_ms(float *ms, void *start, void *end);
 int bw_hip_compute_division(uint64_t count, uint64_t *threads_per_block,
                             uint64_t *blocks);
 
-/* Streaming copy kernel of n 32-bit words (dst[i] = src[i]); used to
- * calibrate the HBM byte counters (FETCH_SIZE / WRITE_SIZE). */
-int bw_hip_copy_kernel(uint32_t *dst_device, const uint32_t *src_device,
-                       uint64_t n, void *stream);
-
-/* Gather-rate probe: out[t] = sum over r of table[idx[r * threads + t]]
- * (indices >= table_len read 0), 16 loads in flight per thread, so the
- * lanes of one wave read consecutive index entries.  n must be a positive
- * multiple of 16 * threads, threads a multiple of 256, out holds `threads`
- * floats.  Measures the L2 request ceiling that bounds the PageRank
- * gathers; the launch time goes to *kernel_ms when it is not NULL. */
-int bw_hip_gather_probe(const uint32_t *idx_device, uint64_t n,
-                        const float *table_device, uint32_t table_len,
-                        float *out_device, uint32_t threads, float *kernel_ms,
-                        void *stream);
-
 /* ------------------------------------------------------- launch config */
 
 /* Launch configuration of one kernel (setWorkSizes, CUDA.cpp:195-241, plus

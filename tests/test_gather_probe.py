@@ -7,6 +7,16 @@ import pytest
 from belewitte_amd import profiling
 
 
+def test_probe_library_is_separate_from_the_kernel_abi():
+    """The probes are exported by libbw_probes.so only, not by the kernel
+    library whose header lists the path's entry points."""
+    from belewitte_amd import _native as N
+    lib = profiling.probes_lib()
+    assert hasattr(lib, "bw_probe_copy") and hasattr(lib, "bw_probe_gather")
+    for name in ("bw_hip_copy_kernel", "bw_hip_gather_probe", "bw_probe_copy"):
+        assert not hasattr(N.hip_lib(), name)
+
+
 def test_indices_share_lines():
     idx = profiling.gather_indices(1 << 12, 1 << 16, 4, seed=3)
     assert idx.max() < (1 << 16)
@@ -20,7 +30,6 @@ def test_indices_share_lines():
 
 @pytest.mark.gpu
 def test_probe_sums_match_numpy(gpu):
-    from belewitte_amd import _native as N
     from belewitte_amd.device import DeviceArray
     threads, n, tlen = 256 * 4, 256 * 4 * 16 * 3, 4096
     table = (np.arange(tlen) % 7).astype(np.float32)
@@ -30,14 +39,15 @@ def test_probe_sums_match_numpy(gpu):
     d_t.upload(table)
     d_i.upload(idx)
     ms = C.c_float()
-    N.check_hip(N.hip_lib().bw_hip_gather_probe(d_i.ptr, n, d_t.ptr, tlen, d_o.ptr, threads, C.byref(ms), None))
+    lib = profiling.probes_lib()
+    profiling.check_probe(lib.bw_probe_gather(d_i.ptr, n, d_t.ptr, tlen, d_o.ptr, threads, C.byref(ms), None))
     got = d_o.download()[:threads]
     vals = np.where(idx < tlen, table[np.minimum(idx, tlen - 1)], 0.0).astype(np.float64)
     want = vals.reshape(-1, threads).sum(axis=0)  # entry r * threads + t belongs to thread t
     assert np.array_equal(got.astype(np.float64), want)
     assert ms.value > 0
     # bad shapes are refused before launch
-    assert N.hip_lib().bw_hip_gather_probe(d_i.ptr, n + 1, d_t.ptr, tlen, d_o.ptr, threads, None, None) != 0
-    assert N.hip_lib().bw_hip_gather_probe(d_i.ptr, n, d_t.ptr, tlen, d_o.ptr, 100, None, None) != 0
+    assert lib.bw_probe_gather(d_i.ptr, n + 1, d_t.ptr, tlen, d_o.ptr, threads, None, None) != 0
+    assert lib.bw_probe_gather(d_i.ptr, n, d_t.ptr, tlen, d_o.ptr, 100, None, None) != 0
     for a in (d_t, d_i, d_o):
         a.free()

@@ -9,6 +9,7 @@ import numpy as np
 import pytest
 
 from belewitte_amd import graph
+from graphfiles import duplicate_edge_graph
 
 ROOT = Path(__file__).resolve().parents[1]
 
@@ -170,3 +171,21 @@ def test_reorder_by_degree():
     assert np.all(np.diff(d.astype(np.int64)) <= 0)
     assert r.edge_count == g.edge_count
     assert np.array_equal(np.sort(perm), np.arange(g.vertex_count))
+
+
+def test_reorder_keeps_duplicate_edges(tmp_path):
+    """The degree relabelling is a pure permutation: a file with duplicate
+    edges keeps its edge count and its edge multiset through load and
+    reorder_by_degree, in both stored directions."""
+    p = tmp_path / "dup.graph"
+    src, dst = duplicate_edge_graph(p)
+    g = graph.load(p)
+    assert g.edge_count == src.size
+    r, perm = g.reorder_by_degree()
+    assert r.edge_count == g.edge_count
+    want = sorted(zip(perm[src].tolist(), perm[dst].tolist()))
+    s, c = r.edges(False)
+    assert sorted(zip(s.tolist(), c.tolist())) == want
+    rd, rs = r.edges(True)
+    assert sorted(zip(rs.tolist(), rd.tolist())) == want
+    assert np.all(np.diff(r.out_degrees().astype(np.int64)) <= 0)

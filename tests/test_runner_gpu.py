@@ -197,3 +197,23 @@ def test_fused_pagerank_through_the_runner(gfile, tmp_path):
     assert sorted(order) == list(range(g.vertex_count))
     timings = (tmp_path / "rmat11.timings").read_text().splitlines() if (tmp_path / "rmat11.timings").exists() else []
     assert all(TIMING.match(t) for t in timings)
+
+
+def test_fused_matches_pull_nodiv_on_duplicate_edges(tmp_path, gpu):
+    """A file with duplicate edges: the fused implementation (relabelled by
+    degree on load, binned by default) ranks the same multigraph as
+    vertex-pull-nodiv, within 1e-6 plus the reference path's own error."""
+    from graphfiles import duplicate_edge_graph
+    p = tmp_path / "dup.graph"
+    duplicate_edge_graph(p)
+    g = graph.load(p)
+    csr = O.CSR.from_graph(g)
+    out = {}
+    for k in ("vertex-pull-nodiv", "vertex-pull-nodiv-fused"):
+        run(["-a", "pagerank", "-k", k, "--validate", "-o", str(tmp_path), str(p)])
+        out[k] = np.array([float(l.split("\t")[1]) for l in (tmp_path / "dup.output").read_text().splitlines()])
+    want, _ = O.pagerank_f32(csr, pagerank.variant_id("vertex-pull-nodiv"), 100, "drop")
+    x = O.pagerank_f64(csr, 100, "drop", nodiv_quirk=True)
+    ref_err = float(np.max(np.abs(want.astype(np.float64) - x)))
+    assert out["vertex-pull-nodiv"].size == g.vertex_count == out["vertex-pull-nodiv-fused"].size
+    assert float(np.max(np.abs(out["vertex-pull-nodiv-fused"] - out["vertex-pull-nodiv"]))) <= 1e-6 + ref_err
