@@ -132,6 +132,7 @@ _HIP_SIGS = {
     "bw_bfs_level": (C.c_int, [_G, C.c_int32, C.c_int32, vp, C.c_int32, vp, _L]),
     "bw_build_edge_lists": (C.c_int, [C.c_uint64, C.c_uint64, vp, C.c_int32, vp, vp, vp, vp, vp]),
     "bw_bfs_run": (C.c_int, [_G, i32p, C.c_int32, C.c_int32, C.c_uint32, vp, u32p, f32p, C.c_int64, i64p, _L]),
+    "bw_bfs_run_static": (C.c_int, [_G, i32p, C.c_int32, C.c_int32, C.c_uint32, vp, u32p, C.c_int64, i64p, _L]),
     # PageRank
     "bw_pr_zero_degrees": (C.c_int, [C.c_uint64, vp, _L]),
     "bw_pr_degrees_edge_list": (C.c_int, [C.c_uint64, vp, vp, _L]),

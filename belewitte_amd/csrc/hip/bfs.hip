@@ -11,6 +11,7 @@
 #include "common.hpp"
 
 #include <algorithm>
+#include <mutex>
 #include <vector>
 
 namespace bw {
@@ -565,4 +566,79 @@ extern "C" int bw_bfs_run(const bw_graph_dev *g, const int32_t *schedule, int32_
     *num_levels = curr;
     cleanup();
     return status;
+}
+
+// ----------------------------------------------- allocation-free level loop
+
+// Scratch of bw_bfs_run_static: one frontier counter per level, so that a
+// level's count is never reused before it is read.  Each device holds its
+// own copy; its address comes from hipGetSymbolAddress, never from the
+// array's name, which is only a host-side stand-in in host code.
+static __device__ __align__(256) uint32_t scratch[BW_BFS_STATIC_MAX_LEVELS];
+
+namespace {
+
+std::mutex g_scratch_mu;
+constexpr int kMaxDevices = 64;
+uint32_t *g_scratch[kMaxDevices] = {};
+
+// Device address of the current device's scratch, looked up once per device.
+int static_scratch(uint32_t **out)
+{
+    int dev = 0;
+    BW_HIP_CHECK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return fail(BW_HIP_ERR_ARG, "device index beyond the scratch table");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch[dev]) {
+        void *p = nullptr;
+        BW_HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        g_scratch[dev] = static_cast<uint32_t *>(p);
+    }
+    *out = g_scratch[dev];
+    return BW_HIP_OK;
+}
+
+}  // namespace
+
+extern "C" int bw_bfs_run_static(const bw_graph_dev *g, const int32_t *schedule, int32_t schedule_len,
+                                 int32_t reduction, uint32_t root, int32_t *levels, uint32_t *counts_host,
+                                 int64_t max_levels, int64_t *num_levels, const bw_launch *cfg)
+{
+    if (!g || !schedule || schedule_len <= 0 || !levels || !num_levels)
+        return fail(BW_HIP_ERR_ARG, "bw_bfs_run_static: invalid argument");
+    if (max_levels < 0 || max_levels > BW_BFS_STATIC_MAX_LEVELS)
+        return fail(BW_HIP_ERR_ARG, "bw_bfs_run_static: max_levels " + std::to_string(max_levels) +
+                                        " exceeds the " + std::to_string(BW_BFS_STATIC_MAX_LEVELS) +
+                                        " frontier counters of the static scratch");
+    *num_levels = 0;
+    if (root >= g->vertex_count) return BW_HIP_OK;  // bfs/bfs.cpp:53
+    hipStream_t s = stream_of(cfg);
+    uint32_t *counters = nullptr;
+    BW_TRY(static_scratch(&counters));
+    // The counters of this search start at zero (a fill kernel, no memset).
+    if (max_levels > 0) BW_TRY(bw_hip_fill_u32(counters, 0u, static_cast<uint64_t>(max_levels), s));
+    BW_TRY(bw_hip_fill_u32(levels, static_cast<uint32_t>(kUnvisited), g->vertex_count, s));
+    hipLaunchKernelGGL(set_root_kernel, dim3(1), dim3(1), 0, s, levels, root);
+    BW_LAUNCH_CHECK();
+    int64_t curr = 0;
+    uint32_t count = 1;
+    while (count != 0) {
+        if (curr >= max_levels) {
+            *num_levels = curr;
+            return fail(BW_HIP_ERR_LIMIT, "BFS did not finish within max_levels launches");
+        }
+        const int32_t variant = schedule[curr % schedule_len];
+        const int st = bw_bfs_level(g, variant, reduction, levels, static_cast<int32_t>(curr), counters + curr, cfg);
+        if (st != BW_HIP_OK) {
+            *num_levels = curr;
+            return st;
+        }
+        // getFrontier (bfs/bfs.cu:13-18): the read-back ends the level.
+        BW_HIP_CHECK(hipMemcpyAsync(&count, counters + curr, sizeof count, hipMemcpyDeviceToHost, s));
+        BW_HIP_CHECK(hipStreamSynchronize(s));
+        if (counts_host) counts_host[curr] = count;
+        curr++;
+    }
+    *num_levels = curr;
+    return BW_HIP_OK;
 }

@@ -238,6 +238,24 @@ int bw_bfs_run(const bw_graph_dev *g, const int32_t *schedule,
                int32_t *levels, uint32_t *counts_host, float *level_ms_host,
                int64_t max_levels, int64_t *num_levels, const bw_launch *cfg);
 
+/* No counterpart in the reference: bw_bfs_run for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  Same level loop, schedule,
+ * results and errors, but it calls no hipMalloc/hipFree and creates no
+ * event: the frontier counters, one uint32 per level, live in a static
+ * device array of BW_BFS_STATIC_MAX_LEVELS entries (256 KiB, held by each
+ * GPU the library runs on until the process ends), cleared by a fill
+ * kernel at the start of each call.  max_levels above that bound is
+ * rejected with BW_HIP_ERR_ARG before any launch; there is no per-level
+ * timing.  Every call on one GPU shares its array, so calls that overlap
+ * on one GPU (other host threads, other streams) race. */
+#define BW_BFS_STATIC_MAX_LEVELS 65536
+int bw_bfs_run_static(const bw_graph_dev *g, const int32_t *schedule,
+                      int32_t schedule_len, int32_t reduction, uint32_t root,
+                      int32_t *levels, uint32_t *counts_host,
+                      int64_t max_levels, int64_t *num_levels,
+                      const bw_launch *cfg);
+
 /* ---------------------------------------------------------- PageRank */
 
 /* Degree kernels, pagerank/pagerank.cu:19-123.  deg is device uint32[V]. */

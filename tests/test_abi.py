@@ -58,6 +58,12 @@ def test_host_side_argument_checks():
     assert lib.bw_bfs_level(C.byref(g), 42, 0, C.c_void_p(1), 0, C.c_void_p(1), None) == 1
     assert b"unknown BFS variant" in lib.bw_hip_last_error()
     assert lib.bw_bfs_run(None, None, 0, 0, 0, None, None, None, 0, None, None) == 1
+    # The allocation-free loop refuses more levels than its static counters
+    # hold, before touching the device.
+    sched = (C.c_int32 * 1)(4)
+    n = C.c_int64(0)
+    assert lib.bw_bfs_run_static(C.byref(g), sched, 1, 0, 0, C.c_void_p(1), None, 65537, C.byref(n), None) == 1
+    assert b"static scratch" in lib.bw_hip_last_error()
 
 
 def test_version_string():
