@@ -153,6 +153,13 @@ _HIP_SIGS = {
     "bw_pr_dangling_mass": (C.c_int, [C.c_uint64, vp, vp, vp, _L]),
     "bw_pr_predivide": (C.c_int, [C.c_uint64, vp, vp, _L]),
     "bw_pr_run": (C.c_int, [_G, C.c_int32, C.c_int32, C.c_int32, vp, f64p, _L]),
+    "bw_pr_pull_plan_create": (C.c_int, [C.POINTER(vp), C.c_uint64, vp, C.c_int32, vp]),
+    "bw_pr_pull_plan_destroy": (C.c_int, [vp]),
+    "bw_pr_pull_plan_info": (C.c_int, [vp, u32p]),
+    "bw_pr_pull_iteration": (C.c_int, [vp, C.c_int32, vp, vp, vp, vp, _L]),
+    "bw_pr_scatter_create": (C.c_int, [C.POINTER(vp), C.c_int32, C.c_uint64, C.c_uint64, vp, vp, C.c_int32, vp]),
+    "bw_pr_scatter_destroy": (C.c_int, [vp]),
+    "bw_pr_scatter_iteration": (C.c_int, [vp, vp, vp, vp, vp]),
     # fused pull PageRank
     "bw_pr_plan_create": (C.c_int, [C.POINTER(vp), C.c_uint64, C.c_uint64, vp, C.c_int32, vp]),
     "bw_pr_plan_destroy": (C.c_int, [vp]),

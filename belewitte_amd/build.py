@@ -40,8 +40,8 @@ HIP_FLAGS = [
     f"-I{INC}",
 ]
 
-HIP_SOURCES = ["backend.hip", "bfs.hip", "pagerank.hip", "bfs_native.hip", "pagerank_native.hip", "mlp.hip",
-               "msbfs.hip"]
+HIP_SOURCES = ["backend.hip", "bfs.hip", "pagerank.hip", "pagerank_scatter.hip", "bfs_native.hip",
+               "pagerank_native.hip", "mlp.hip", "msbfs.hip"]
 
 
 def _run(cmd, cwd=None):

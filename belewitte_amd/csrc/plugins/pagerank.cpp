@@ -99,9 +99,66 @@ void reverseCSRComputeDegrees(CSR<unsigned, unsigned> *g, unsigned *degrees)
     BW_CHK(bw_pr_degrees_rev_csr(g->vertex_count, g->vertices, 4, g->edges, degrees, &c));
 }
 
+// Binned layouts of the push forms (bw_pr_scatter_*), one per source kind,
+// built on a run's first iteration and released when the next run starts;
+// a graph the layout refuses keeps the direct atomic kernels.
+struct RunScatter {
+    bw_pr_scatter *layout = nullptr;
+    const void *key = nullptr;
+    bool refused = false;
+};
+RunScatter runScatter[3];
+
+// The pull plan of the run (long rows one wave each), built on the first
+// pull iteration of a run for the reverse CSR it is given.
+bw_pr_pull_plan *runPull = nullptr;
+const void *runPullKey = nullptr;
+
+void resetScatter()
+{
+    for (auto &r : runScatter) {
+        if (r.layout) BW_CHK(bw_pr_scatter_destroy(r.layout));
+        r = RunScatter{};
+    }
+    if (runPull) BW_CHK(bw_pr_pull_plan_destroy(runPull));
+    runPull = nullptr;
+    runPullKey = nullptr;
+}
+
+bw_pr_pull_plan *pullPlanFor(CSR<unsigned, unsigned> *g)
+{
+    if (runPullKey != g->vertices) {
+        if (runPull) BW_CHK(bw_pr_pull_plan_destroy(runPull));
+        runPull = nullptr;
+        BW_CHK(bw_pr_pull_plan_create(&runPull, g->vertex_count, g->vertices, 4, nullptr));
+        runPullKey = g->vertices;
+    }
+    return runPull;
+}
+
+bw_pr_scatter *scatterFor(int32_t kind, uint64_t V, uint64_t E, const void *a, const void *b)
+{
+    RunScatter &r = runScatter[kind];
+    if (r.key != a) {
+        if (r.layout) BW_CHK(bw_pr_scatter_destroy(r.layout));
+        r = RunScatter{};
+        r.key = a;
+        const int st = bw_pr_scatter_create(&r.layout, kind, V, E, a, b, 4, nullptr);
+        if (st == BW_HIP_ERR_ARG) r.refused = true;
+        else BW_CHK(st);
+    }
+    return r.refused ? nullptr : r.layout;
+}
+
 // Iteration kernels.
 void edgeListPageRank(EdgeList<unsigned> *g, unsigned *deg, float *pr, float *np)
 {
+    if (auto *sc = g->edge_count ? scatterFor(BW_PR_SCATTER_EDGE_LIST, g->vertex_count, g->edge_count, g->inEdges,
+                                              g->outEdges)
+                                 : nullptr) {
+        BW_CHK(bw_pr_scatter_iteration(sc, deg, pr, np, nullptr));
+        return;
+    }
     auto c = config();
     BW_CHK(bw_pr_edge_list(g->edge_count, g->inEdges, g->outEdges, deg, pr, np, &c));
 }
@@ -114,6 +171,11 @@ void revEdgeListPageRank(EdgeList<unsigned> *g, unsigned *deg, float *pr, float 
 
 void structEdgeListPageRank(StructEdgeList<unsigned> *g, unsigned *deg, float *pr, float *np)
 {
+    if (auto *sc = g->edge_count ? scatterFor(BW_PR_SCATTER_STRUCT, g->vertex_count, g->edge_count, g->edges, nullptr)
+                                 : nullptr) {
+        BW_CHK(bw_pr_scatter_iteration(sc, deg, pr, np, nullptr));
+        return;
+    }
     auto c = config();
     BW_CHK(bw_pr_struct_edge_list(g->edge_count, reinterpret_cast<const uint32_t *>(g->edges), deg, pr, np, &c));
 }
@@ -126,6 +188,11 @@ void revStructEdgeListPageRank(StructEdgeList<unsigned> *g, unsigned *deg, float
 
 void vertexPushPageRank(CSR<unsigned, unsigned> *g, unsigned *, float *pr, float *np)
 {
+    if (auto *sc = g->edge_count ? scatterFor(BW_PR_SCATTER_CSR, g->vertex_count, g->edge_count, g->vertices, g->edges)
+                                 : nullptr) {
+        BW_CHK(bw_pr_scatter_iteration(sc, nullptr, pr, np, nullptr));
+        return;
+    }
     auto c = config();
     BW_CHK(bw_pr_vertex_push(g->vertex_count, g->vertices, 4, g->edges, pr, np, &c));
 }
@@ -133,7 +200,7 @@ void vertexPushPageRank(CSR<unsigned, unsigned> *g, unsigned *, float *pr, float
 void vertexPullPageRank(CSR<unsigned, unsigned> *g, unsigned *deg, float *pr, float *np)
 {
     auto c = config();
-    BW_CHK(bw_pr_vertex_pull(g->vertex_count, g->vertices, 4, g->edges, deg, pr, np, &c));
+    BW_CHK(bw_pr_pull_iteration(pullPlanFor(g), BW_PR_VERTEX_PULL, g->edges, deg, pr, np, &c));
 }
 
 void predivideOnce(uint64_t V, const unsigned *deg, float *pr)
@@ -147,11 +214,18 @@ void vertexPullNoDivPageRank(CSR<unsigned, unsigned> *g, unsigned *deg, float *p
 {
     predivideOnce(g->vertex_count, deg, pr);
     auto c = config();
-    BW_CHK(bw_pr_vertex_pull_nodiv(g->vertex_count, g->vertices, 4, g->edges, pr, np, &c));
+    BW_CHK(bw_pr_pull_iteration(pullPlanFor(g), BW_PR_VERTEX_PULL_NODIV, g->edges, deg, pr, np, &c));
 }
 
 void vertexPushWarpPageRank(size_t w, size_t ch, CSR<unsigned, unsigned> *g, unsigned *, float *pr, float *np)
 {
+    // The warp geometry only shapes the atomic kernel; the binned layout
+    // does not depend on it.
+    if (auto *sc = g->edge_count ? scatterFor(BW_PR_SCATTER_CSR, g->vertex_count, g->edge_count, g->vertices, g->edges)
+                                 : nullptr) {
+        BW_CHK(bw_pr_scatter_iteration(sc, nullptr, pr, np, nullptr));
+        return;
+    }
     auto c = config(w, ch);
     BW_CHK(bw_pr_vertex_push_warp(g->vertex_count, g->vertices, 4, g->edges, pr, np, &c));
 }
@@ -159,7 +233,7 @@ void vertexPushWarpPageRank(size_t w, size_t ch, CSR<unsigned, unsigned> *g, uns
 void vertexPullWarpPageRank(size_t w, size_t ch, CSR<unsigned, unsigned> *g, unsigned *deg, float *pr, float *np)
 {
     auto c = config(w, ch);
-    BW_CHK(bw_pr_vertex_pull_warp(g->vertex_count, g->vertices, 4, g->edges, deg, pr, np, &c));
+    BW_CHK(bw_pr_pull_iteration(pullPlanFor(g), BW_PR_VERTEX_PULL_WARP, g->edges, deg, pr, np, &c));
 }
 
 void vertexPullWarpNoDivPageRank(size_t w, size_t ch, CSR<unsigned, unsigned> *g, unsigned *deg, float *pr,
@@ -167,7 +241,7 @@ void vertexPullWarpNoDivPageRank(size_t w, size_t ch, CSR<unsigned, unsigned> *g
 {
     predivideOnce(g->vertex_count, deg, pr);
     auto c = config(w, ch);
-    BW_CHK(bw_pr_vertex_pull_warp_nodiv(g->vertex_count, g->vertices, 4, g->edges, pr, np, &c));
+    BW_CHK(bw_pr_pull_iteration(pullPlanFor(g), BW_PR_VERTEX_PULL_WARP_NODIV, g->edges, deg, pr, np, &c));
 }
 
 // Consolidate kernels (pagerank/pagerank.cu:125-165).
@@ -310,6 +384,9 @@ struct PageRank : public ImplementationTemplate<Platform, Vertex, Edge, switchin
             // UNIFORM mode starts the nodiv variants from divided
             // contributions; their launchers apply it on the first call.
             predividePending = danglingMode == BW_PR_DANGLING_UNIFORM;
+            // Each run builds its push layouts afresh (inside the
+            // computation timer, as the degree kernels are).
+            resetScatter();
 
             int j = 0;
             do {
@@ -332,6 +409,7 @@ struct PageRank : public ImplementationTemplate<Platform, Vertex, Edge, switchin
             resultTransfer.stop();
         }
 
+        resetScatter();
         writeRanks(outputFile, pageranks, pageranks.size, validate);
     }
 };

@@ -325,6 +325,60 @@ int bw_pr_vertex_pull_warp_nodiv(uint64_t vertex_count, const void *rev_offsets,
                                  const float *pr, float *new_pr,
                                  const bw_launch *cfg);
 
+/* Pull plan of the four pull variants (vertexPullPageRank,
+ * vertexPullNoDivPageRank, vertexPullWarpPageRank,
+ * vertexPullWarpNoDivPageRank; pagerank/vertex_pull*.cu): the rows of the
+ * reverse CSR with more than 2048 in-edges, listed once per graph on the
+ * device, are summed one wave each, so a skewed graph's hubs run side by
+ * side; every other row is summed by the workgroup owning it.  Results are
+ * those of bw_pr_vertex_pull* (the non-warp variants bit for bit: each row
+ * is still a sequential FP32 sum in CSR order).  The reverse offsets must
+ * stay live while the plan is used. */
+typedef struct bw_pr_pull_plan bw_pr_pull_plan;
+int bw_pr_pull_plan_create(bw_pr_pull_plan **plan, uint64_t vertex_count,
+                           const void *rev_offsets, int32_t offset_bytes,
+                           void *stream);
+int bw_pr_pull_plan_destroy(bw_pr_pull_plan *plan);
+int bw_pr_pull_plan_info(const bw_pr_pull_plan *plan, uint32_t *long_rows);
+/* One iteration of `variant` (BW_PR_VERTEX_PULL, _NODIV, _WARP,
+ * _WARP_NODIV; cfg->warp is the virtual warp of the _warp variants). */
+int bw_pr_pull_iteration(const bw_pr_pull_plan *plan, int32_t variant,
+                         const uint32_t *rev_cols, const uint32_t *deg,
+                         const float *pr, float *new_pr, const bw_launch *cfg);
+
+/* Propagation blocking for the push forms (edgeListPageRank,
+ * structEdgeListPageRank, vertexPushPageRank, vertexPushWarpPageRank of
+ * pagerank/edgelist.cu:3-23, struct_edgelist.cu:3-24, vertex_push.cu:3-31,
+ * vertex_push_warp.cu:4-49).  Those add each edge's contribution to
+ * new_pr[destination] with a global atomic; here a layout built once per
+ * graph (bw_pr_scatter_create: one pass over the destinations and a scan,
+ * 6 bytes of bins per edge) lets one iteration bin every (destination,
+ * contribution) by partition of 8192 rows and sum each partition in LDS
+ * (FP64), writing each row once per chunk of its bin.  Sums are added in a
+ * different order than the reference's atomics, which have no fixed order
+ * either.  Source kinds: the forward EdgeList (a = in[], b = out[]), the
+ * forward StructEdgeList (a = the (in, out) pairs, b unused) or the forward
+ * CSR (a = offsets of offset_bytes, b = columns; the contribution is
+ * pr[v] / outdeg(v) from the CSR itself, as vertex_push.cu does; each
+ * edge's source row is expanded once when the layout is built, 4 more
+ * bytes per edge).  The graph arrays must stay live while the layout is
+ * used.  Graphs beyond
+ * 2^25 vertices or 2^32 - 1 edges are refused with BW_HIP_ERR_ARG. */
+#define BW_PR_SCATTER_EDGE_LIST 0
+#define BW_PR_SCATTER_STRUCT 1
+#define BW_PR_SCATTER_CSR 2
+typedef struct bw_pr_scatter bw_pr_scatter;
+int bw_pr_scatter_create(bw_pr_scatter **scatter, int32_t kind,
+                         uint64_t vertex_count, uint64_t edge_count,
+                         const void *a, const void *b, int32_t offset_bytes,
+                         void *stream);
+int bw_pr_scatter_destroy(bw_pr_scatter *scatter);
+/* One iteration kernel: new_pr[v] += the contributions of v's in-edges
+ * (new_pr zeroed by the previous consolidate, as the reference assumes).
+ * deg (out-degrees) is needed by the edge-list kinds only. */
+int bw_pr_scatter_iteration(const bw_pr_scatter *scatter, const uint32_t *deg,
+                            const float *pr, float *new_pr, void *stream);
+
 /* PageRank modes. */
 #define BW_PR_DANGLING_DROP 0    /* reference: dangling mass is lost        */
 #define BW_PR_DANGLING_UNIFORM 1 /* dangling mass redistributed uniformly   */

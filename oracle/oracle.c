@@ -287,15 +287,20 @@ static void pr_iteration(const oracle_graph *g, int variant, uint32_t warp,
         break;
     case ORACLE_PR_REV_EDGE_LIST:
     case ORACLE_PR_REV_STRUCT_EDGE_LIST:
-        /* pagerank/rev_edgelist.cu:14-22: origin = reverse out[] (source) */
-        for (uint64_t row = 0; row < V; row++)
+        /* pagerank/rev_edgelist.cu:14-22: origin = reverse out[] (source).
+         * Each row's sum is sequential, so rows run in parallel without
+         * changing a bit. */
+#pragma omp parallel for schedule(dynamic, 1024)
+        for (int64_t r = 0; r < (int64_t)V; r++) {
+            const uint64_t row = (uint64_t)r;
             for (uint64_t j = g->roff[row]; j < g->roff[row + 1]; j++) {
                 uint32_t src = g->rcol[j];
                 uint32_t d = deg[src];
-                float r = 0.0f;
-                if (d != 0) r = pr[src] / (float)d;
-                np[row] += r;
+                float c = 0.0f;
+                if (d != 0) c = pr[src] / (float)d;
+                np[row] += c;
             }
+        }
         break;
     case ORACLE_PR_VERTEX_PUSH:
         /* pagerank/vertex_push.cu:16-30 (degree from the CSR itself) */
@@ -318,8 +323,10 @@ static void pr_iteration(const oracle_graph *g, int variant, uint32_t warp,
         }
         break;
     case ORACLE_PR_VERTEX_PULL:
-        /* pagerank/vertex_pull.cu:16-30: sequential in-edge sum */
-        for (uint64_t v = 0; v < V; v++) {
+        /* pagerank/vertex_pull.cu:16-30: sequential in-edge sum (rows in
+         * parallel, each row's order unchanged) */
+#pragma omp parallel for schedule(dynamic, 1024)
+        for (int64_t v = 0; v < (int64_t)V; v++) {
             float acc = 0.0f;
             for (uint64_t j = g->roff[v]; j < g->roff[v + 1]; j++) {
                 uint32_t u = g->rcol[j];
@@ -330,7 +337,8 @@ static void pr_iteration(const oracle_graph *g, int variant, uint32_t warp,
         break;
     case ORACLE_PR_VERTEX_PULL_NODIV:
         /* pagerank/vertex_pull_nodiv.cu:16-28 */
-        for (uint64_t v = 0; v < V; v++) {
+#pragma omp parallel for schedule(dynamic, 1024)
+        for (int64_t v = 0; v < (int64_t)V; v++) {
             float acc = 0.0f;
             for (uint64_t j = g->roff[v]; j < g->roff[v + 1]; j++) acc += pr[g->rcol[j]];
             np[v] = acc;
@@ -340,7 +348,8 @@ static void pr_iteration(const oracle_graph *g, int variant, uint32_t warp,
     case ORACLE_PR_VERTEX_PULL_WARP_NODIV:
         /* pagerank/vertex_pull_warp.cu:36-47 and _nodiv.cu:35-45: per-lane
          * partial sums, each atomically added to np[v] in lane order. */
-        for (uint64_t v = 0; v < V; v++) {
+#pragma omp parallel for schedule(dynamic, 1024)
+        for (int64_t v = 0; v < (int64_t)V; v++) {
             uint64_t s = g->roff[v], n = g->roff[v + 1] - s;
             for (uint32_t lane = 0; lane < warp; lane++) {
                 float part = 0.0f;

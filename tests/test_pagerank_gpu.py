@@ -18,11 +18,23 @@ TOL = 1e-6
 BIT_EXACT = {"vertex-pull", "vertex-pull-nodiv"}
 
 
+def _hubs():
+    """An R-MAT with an in-hub of 20000 in-edges (vertex 0) and an out-hub of
+    9000 out-edges (vertex 1): rows past the pull kernels' 2048-edge split
+    and a source whose push spans many bin workgroups."""
+    r = graph.rmat(15, 4, seed=41)
+    s, d = r.edges()
+    n = r.vertex_count
+    extra_s = np.concatenate([np.arange(2, 20002, dtype=np.uint32) % n, np.full(9000, 1, np.uint32)])
+    extra_d = np.concatenate([np.zeros(20000, np.uint32), np.arange(5, 9005, dtype=np.uint32)])
+    return graph.from_edges(n, np.concatenate([s, extra_s]), np.concatenate([d, extra_d]))
+
+
 @pytest.fixture(scope="module")
 def cases(gpu):
     out = []
     for g in (graph.rmat(12, 16, seed=31), graph.erdos_renyi(2000, 9000, seed=3, undirected=False),
-              graph.rmat(11, 8, seed=32, undirected=True), graph.star(300), graph.chain(50)):
+              graph.rmat(11, 8, seed=32, undirected=True), graph.star(300), graph.chain(50), _hubs()):
         out.append((g, O.CSR.from_graph(g), device.DeviceGraph(g, reps=device.REPRESENTATIONS)))
     yield out
     for _, _, dg in out:
@@ -37,24 +49,33 @@ def test_variant_matches_oracle(cases, variant, mode, iters):
         print(f"case {g.name} variant={variant} mode={mode}", flush=True)
         ranks, resid = pagerank.run(dg, variant, iterations=iters, mode=mode)
         want, want_resid = O.pagerank_f32(csr, vid, iters, mode)
+        x = O.pagerank_f64(csr, iters, mode, nodiv_quirk=(vid in pagerank.NODIV and mode == "drop"))
         err = float(np.max(np.abs(ranks.astype(np.float64) - want.astype(np.float64))))
-        assert err <= TOL, (g.name, variant, mode, err)
+        err64 = float(np.max(np.abs(ranks - x)))
+        # The oracle's own error against FP64: FP32 rounding of sequential
+        # sums, which grows on hub rows (about 1e-5 on a 20000-edge row).
+        ref_err = float(np.max(np.abs(want.astype(np.float64) - x)))
         if variant in BIT_EXACT:
             assert np.array_equal(ranks.view(np.uint32), want.view(np.uint32)), (g.name, variant, mode)
+        else:
+            # Sums without a fixed order (atomic in the reference): 1e-6 of
+            # the oracle, widened by the oracle's own error.
+            assert err <= TOL + ref_err, (g.name, variant, mode, err, ref_err)
+        # Never further from FP64 than the reference path is (plus 1e-6).
+        assert err64 <= ref_err + TOL, (g.name, variant, mode, err64, ref_err)
         assert np.all(np.isfinite(resid))
-        # FP64 oracle: error at FP32 rounding level (relative 1e-5 of the mass).
-        x = O.pagerank_f64(csr, iters, mode, nodiv_quirk=(vid in pagerank.NODIV and mode == "drop"))
-        assert float(np.max(np.abs(ranks - x))) <= 1e-5 * max(float(np.abs(x).max()), 1e-30) + 1e-9
 
 
 @pytest.mark.parametrize("warp,chunk", [(1, 4), (8, 16), (64, 64)])
 @pytest.mark.parametrize("variant", ["vertex-push-warp", "vertex-pull-warp", "vertex-pull-warp-nodiv"])
 def test_warp_geometries(cases, variant, warp, chunk):
-    g, csr, dg = cases[0]
     vid = pagerank.variant_id(variant)
-    ranks, _ = pagerank.run(dg, variant, iterations=20, mode="drop", warp=warp, chunk=chunk)
-    want, _ = O.pagerank_f32(csr, vid, 20, "drop", warp=warp)
-    assert float(np.max(np.abs(ranks - want))) <= TOL
+    for g, csr, dg in (cases[0], cases[-1]):  # plain R-MAT, and the hubs past the long-row split
+        ranks, _ = pagerank.run(dg, variant, iterations=20, mode="drop", warp=warp, chunk=chunk)
+        want, _ = O.pagerank_f32(csr, vid, 20, "drop", warp=warp)
+        x = O.pagerank_f64(csr, 20, "drop", nodiv_quirk=vid in pagerank.NODIV)
+        ref_err = float(np.max(np.abs(want.astype(np.float64) - x)))
+        assert float(np.max(np.abs(ranks - want))) <= TOL + ref_err, (g.name, variant, warp)
 
 
 def test_uniform_mode_conserves_mass(cases):
