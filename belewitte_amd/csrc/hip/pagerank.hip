@@ -38,20 +38,30 @@ __device__ __forceinline__ void add_f64(double *p, double v) { unsafeAtomicAdd(p
 
 // ---------------------------------------------------------------- degrees
 
-__global__ __launch_bounds__(1024) void pr_degrees_origins_kernel(uint64_t E, const uint32_t *__restrict__ origins,
-                                                                  uint32_t *deg)
+// Occurrence counts of values[i * STRIDE + off] (the degree kernels'
+// atomicAdd(&deg[x], 1) per edge).  Each wave reads 64 consecutive entries
+// and adds every run of equal values inside them with one atomic from the
+// run's last lane: a source-sorted edge list, whose lanes would otherwise
+// all add to one counter, takes an atomic per run instead of per edge.
+template<int STRIDE>
+__global__ __launch_bounds__(256) void pr_degrees_runs_kernel(uint64_t E, const uint32_t *__restrict__ values,
+                                                              int off, uint32_t *deg)
 {
-    const uint64_t stride = static_cast<uint64_t>(blockDim.x) * gridDim.x;
-    for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < E; i += stride)
-        atomicAdd(&deg[origins[i]], 1u);
-}
-
-__global__ __launch_bounds__(1024) void pr_degrees_struct_kernel(uint64_t E, const uint32_t *__restrict__ edges,
-                                                                 int rev, uint32_t *deg)
-{
-    const uint64_t stride = static_cast<uint64_t>(blockDim.x) * gridDim.x;
-    for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < E; i += stride)
-        atomicAdd(&deg[edges[2 * i + rev]], 1u);
+    const uint32_t lane = threadIdx.x & (kWave - 1);
+    const uint64_t waves = static_cast<uint64_t>(gridDim.x) * (blockDim.x / kWave);
+    for (uint64_t w = (static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x) / kWave; w * kWave < E;
+         w += waves) {
+        const uint64_t i = w * kWave + lane;
+        const bool valid = i < E;
+        const uint32_t x = valid ? values[i * STRIDE + off] : 0xFFFFFFFFu;
+        const uint32_t prev = __shfl_up(x, 1, kWave), next = __shfl_down(x, 1, kWave);
+        const bool head = lane == 0 || prev != x;
+        const bool tail = lane == kWave - 1 || next != x;
+        const uint64_t heads = __ballot(head);
+        const uint64_t upto = lane == kWave - 1 ? ~0ull : ((2ull << lane) - 1ull);
+        const uint32_t h = 63u - static_cast<uint32_t>(__builtin_clzll(heads & upto));
+        if (valid && tail) atomicAdd(&deg[x], lane - h + 1u);
+    }
 }
 
 template<typename Off>
@@ -803,9 +813,9 @@ extern "C" int bw_pr_degrees_edge_list(uint64_t E, const uint32_t *origins, uint
 {
     if (E == 0) return BW_HIP_OK;
     if (!origins || !deg) return fail(BW_HIP_ERR_ARG, "NULL device pointer");
-    uint32_t block, grid;
-    BW_TRY(resolve_geometry(cfg, E, block, grid));
-    hipLaunchKernelGGL(pr_degrees_origins_kernel, dim3(grid), dim3(block), 0, stream_of(cfg), E, origins, deg);
+    uint32_t block = 256, grid = 0;
+    BW_TRY(resolve_geometry(nullptr, E, block, grid));
+    hipLaunchKernelGGL(pr_degrees_runs_kernel<1>, dim3(grid), dim3(block), 0, stream_of(cfg), E, origins, 0, deg);
     BW_LAUNCH_CHECK();
     return BW_HIP_OK;
 }
@@ -815,9 +825,9 @@ extern "C" int bw_pr_degrees_struct_edge_list(uint64_t E, const uint32_t *edges,
 {
     if (E == 0) return BW_HIP_OK;
     if (!edges || !deg) return fail(BW_HIP_ERR_ARG, "NULL device pointer");
-    uint32_t block, grid;
-    BW_TRY(resolve_geometry(cfg, E, block, grid));
-    hipLaunchKernelGGL(pr_degrees_struct_kernel, dim3(grid), dim3(block), 0, stream_of(cfg), E, edges,
+    uint32_t block = 256, grid = 0;
+    BW_TRY(resolve_geometry(nullptr, E, block, grid));
+    hipLaunchKernelGGL(pr_degrees_runs_kernel<2>, dim3(grid), dim3(block), 0, stream_of(cfg), E, edges,
                        reverse ? 1 : 0, deg);
     BW_LAUNCH_CHECK();
     return BW_HIP_OK;
@@ -1280,7 +1290,14 @@ extern "C" int bw_pr_run(const bw_graph_dev *g, int32_t variant, int32_t iterati
     // variants have none in the reference and get them from the CSR only
     // when UNIFORM mode needs the dangling vertices.
     BW_TRY(bw_pr_zero_degrees(V, buf.deg, &cfg));
-    switch (variant) {
+    // With the forward CSR resident, every variant's out-degrees come from
+    // its offsets: the same integers the variant's own degree kernel counts
+    // (pagerank/pagerank.cu:40-123), without an atomic per edge.
+    const bool needDeg = !(variant == BW_PR_VERTEX_PUSH || variant == BW_PR_VERTEX_PUSH_WARP) ||
+                         mode == BW_PR_DANGLING_UNIFORM;
+    if (needDeg && g->offsets) {
+        BW_TRY(bw_pr_degrees_csr(V, g->offsets, g->offset_bytes, buf.deg, &cfg));
+    } else switch (variant) {
     case BW_PR_EDGE_LIST: BW_TRY(bw_pr_degrees_edge_list(E, g->el_in, buf.deg, &cfg)); break;
     case BW_PR_REV_EDGE_LIST: BW_TRY(bw_pr_degrees_edge_list(E, g->rel_out, buf.deg, &cfg)); break;
     case BW_PR_STRUCT_EDGE_LIST: BW_TRY(bw_pr_degrees_struct_edge_list(E, g->sel, 0, buf.deg, &cfg)); break;

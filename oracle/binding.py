@@ -46,7 +46,8 @@ def lib() -> C.CDLL:
         L.oracle_bfs_run.argtypes = [G, C.c_uint32, C.c_void_p, C.c_int, C.c_uint32, C.c_uint32,
                                      C.c_void_p, C.c_void_p, C.c_int64]
         L.oracle_bfs_vertex_push_omp.restype = C.c_int64
-        L.oracle_bfs_vertex_push_omp.argtypes = [G, C.c_uint32, C.c_void_p, C.c_void_p, C.c_int64, C.c_int]
+        L.oracle_bfs_vertex_push_omp.argtypes = [G, C.c_uint32, C.c_void_p, C.c_void_p, C.c_int64, C.c_int,
+                                                 C.POINTER(C.c_double)]
         L.oracle_pr_out_degrees.restype = None
         L.oracle_pr_out_degrees.argtypes = [G, C.c_void_p]
         L.oracle_pagerank_f32.restype = None
@@ -54,7 +55,7 @@ def lib() -> C.CDLL:
         L.oracle_pagerank_f64.restype = None
         L.oracle_pagerank_f64.argtypes = [G, C.c_int, C.c_int, C.c_int, C.c_void_p]
         L.oracle_pagerank_pull_omp.restype = None
-        L.oracle_pagerank_pull_omp.argtypes = [G, C.c_int, C.c_void_p, C.c_int]
+        L.oracle_pagerank_pull_omp.argtypes = [G, C.c_int, C.c_void_p, C.c_int, C.POINTER(C.c_double)]
         L.oracle_pagerank_bucket_order.restype = None
         L.oracle_pagerank_bucket_order.argtypes = [C.c_void_p, C.c_uint64, C.c_void_p]
         _lib = L
@@ -114,12 +115,17 @@ def bfs_run(csr: CSR, root: int, schedule, warp=32, chunk=32, max_levels=None):
     return levels[:csr.V], counts[:n].copy()
 
 
-def bfs_vertex_push_omp(csr: CSR, root: int, threads: int = 0, max_levels=None):
+def bfs_vertex_push_omp(csr: CSR, root: int, threads: int = 0, max_levels=None, timed: bool = False):
+    """Levels and per-level counts; with timed=True also the seconds of the
+    initialisation and level loop (the reference's computation scope)."""
     max_levels = max_levels or csr.V + 2
     levels = np.empty(max(csr.V, 1), dtype=np.int32)
     counts = np.zeros(max_levels, dtype=np.uint32)
+    sec = C.c_double(0.0)
     n = lib().oracle_bfs_vertex_push_omp(csr.ref, root, levels.ctypes.data, counts.ctypes.data,
-                                         max_levels, threads)
+                                         max_levels, threads, C.byref(sec))
+    if timed:
+        return levels[:csr.V], counts[:n].copy(), sec.value
     return levels[:csr.V], counts[:n].copy()
 
 
@@ -143,9 +149,14 @@ def pagerank_f64(csr: CSR, iterations: int, mode: str = "drop", nodiv_quirk: boo
     return x[:csr.V]
 
 
-def pagerank_pull_omp(csr: CSR, iterations: int, threads: int = 0):
+def pagerank_pull_omp(csr: CSR, iterations: int, threads: int = 0, timed: bool = False):
+    """Ranks; with timed=True also the seconds of the degrees and iterations
+    (the reference's computation scope)."""
     ranks = np.empty(max(csr.V, 1), dtype=np.float32)
-    lib().oracle_pagerank_pull_omp(csr.ref, iterations, ranks.ctypes.data, threads)
+    sec = C.c_double(0.0)
+    lib().oracle_pagerank_pull_omp(csr.ref, iterations, ranks.ctypes.data, threads, C.byref(sec))
+    if timed:
+        return ranks[:csr.V], sec.value
     return ranks[:csr.V]
 
 

@@ -18,10 +18,18 @@
 
 #ifdef _OPENMP
 #include <omp.h>
+#include <time.h>
 #endif
 
 /* Damping constant of pagerank/pagerank.hpp:9 (a float). */
 static const float DAMPENING = 0.85f;
+
+static double now_seconds(void)
+{
+    struct timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return (double)ts.tv_sec + 1e-9 * (double)ts.tv_nsec;
+}
 
 /* ------------------------------------------------------------------ BFS */
 
@@ -213,8 +221,9 @@ int64_t oracle_bfs_run(const oracle_graph *g, uint32_t root, const int *schedule
 
 int64_t oracle_bfs_vertex_push_omp(const oracle_graph *g, uint32_t root,
                                    int32_t *levels, uint32_t *counts,
-                                   int64_t max_levels, int threads)
+                                   int64_t max_levels, int threads, double *seconds)
 {
+    if (seconds) *seconds = 0.0;
     if (root >= g->V) return 0;
     int64_t V = (int64_t)g->V;
 #ifdef _OPENMP
@@ -222,6 +231,9 @@ int64_t oracle_bfs_vertex_push_omp(const oracle_graph *g, uint32_t root,
 #else
     (void)threads;
 #endif
+    /* The "computation" scope of bfs/bfs.cpp:78-108: initialisation and
+     * the level loop. */
+    const double t0 = now_seconds();
 #pragma omp parallel for schedule(static)
     for (int64_t v = 0; v < V; v++) levels[v] = ORACLE_UNVISITED;
     levels[root] = 0;
@@ -248,6 +260,7 @@ int64_t oracle_bfs_vertex_push_omp(const oracle_graph *g, uint32_t root,
         counts[curr] = frontier;
         curr++;
     } while (frontier);
+    if (seconds) *seconds = now_seconds() - t0;
     return curr;
 }
 
@@ -461,7 +474,7 @@ void oracle_pagerank_f64(const oracle_graph *g, int iterations, int mode,
 }
 
 void oracle_pagerank_pull_omp(const oracle_graph *g, int iterations, float *pr,
-                              int threads)
+                              int threads, double *seconds)
 {
     int64_t V = (int64_t)g->V;
 #ifdef _OPENMP
@@ -471,6 +484,9 @@ void oracle_pagerank_pull_omp(const oracle_graph *g, int iterations, float *pr,
 #endif
     uint32_t *deg = (uint32_t *)malloc(sizeof(uint32_t) * (V ? V : 1));
     float *np = (float *)malloc(sizeof(float) * (V ? V : 1));
+    /* The "computation" scope of pagerank/pagerank.cpp:95-127: degrees and
+     * the iterations (buffers are allocated before it). */
+    const double t0 = now_seconds();
     oracle_pr_out_degrees(g, deg);
     float init = 1.0f / (float)V;
 #pragma omp parallel for schedule(static)
@@ -495,6 +511,7 @@ void oracle_pagerank_pull_omp(const oracle_graph *g, int iterations, float *pr,
             pr[v] = (deg[v] != 0 && notLast) ? nr / (float)deg[v] : nr;
         }
     }
+    if (seconds) *seconds = now_seconds() - t0;
     free(deg);
     free(np);
 }

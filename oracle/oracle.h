@@ -89,10 +89,11 @@ int64_t oracle_bfs_run(const oracle_graph *g, uint32_t root, const int *schedule
 
 /* Baseline timing port: level-synchronous vertex-push (bfs/vertex_push.cu)
  * over `threads` OpenMP threads with a CAS atomic-min.  Same outputs as
- * oracle_bfs_run with ORACLE_BFS_VERTEX_PUSH. */
+ * oracle_bfs_run with ORACLE_BFS_VERTEX_PUSH.  *seconds (optional) receives
+ * the monotonic-clock time of the initialisation and level loop. */
 int64_t oracle_bfs_vertex_push_omp(const oracle_graph *g, uint32_t root,
                                    int32_t *levels, uint32_t *counts,
-                                   int64_t max_levels, int threads);
+                                   int64_t max_levels, int threads, double *seconds);
 
 /* ------------------------------------------------------------- PageRank */
 
@@ -121,7 +122,7 @@ void oracle_pagerank_f64(const oracle_graph *g, int iterations, int mode,
  * OpenMP threads; ranks identical to oracle_pagerank_f32 with
  * ORACLE_PR_VERTEX_PULL_NODIV and ORACLE_PR_DANGLING_UNIFORM. */
 void oracle_pagerank_pull_omp(const oracle_graph *g, int iterations,
-                              float *ranks, int threads);
+                              float *ranks, int threads, double *seconds);
 
 /* Bucketed order output of pagerank/pagerank.cpp:142-168: masks the low
  * 16 mantissa bits, stable-sorts (rank, id), then re-sorts ids inside
