@@ -88,7 +88,24 @@ __global__ __launch_bounds__(1024) void bfs_struct_edge_list_kernel(
 
 // ---------------------------------------------------------- vertex forms
 
-// bfs/vertex_push.cu: thread per vertex, expand out-edges of depth-d rows.
+// Out-degree (in-degree for pull) above which a vertex's edges are taken
+// by its whole wave instead of its own thread.
+constexpr uint64_t kWaveDegree = 64;
+
+// The 64-bit offset of lane `l` broadcast to the wave.
+__device__ __forceinline__ uint64_t read_lane_u64(uint64_t x, int l)
+{
+    const uint32_t lo = __builtin_amdgcn_readlane(static_cast<uint32_t>(x), l);
+    const uint32_t hi = __builtin_amdgcn_readlane(static_cast<uint32_t>(x >> 32), l);
+    return (static_cast<uint64_t>(hi) << 32) | lo;
+}
+
+// bfs/vertex_push.cu: a thread per vertex expands the out-edges of its
+// depth-d vertex; a vertex with more than kWaveDegree out-edges is expanded
+// by the whole wave (the lanes stride over its edges), one such vertex at a
+// time in ballot order, so a hub no longer serialises one thread.  Every
+// out-edge of every frontier vertex is relaxed once, as in the reference,
+// so depths and counts are the same.
 template<int R, typename Off>
 __global__ __launch_bounds__(1024) void bfs_vertex_push_kernel(
     uint64_t V, const Off *__restrict__ off, const uint32_t *__restrict__ cols,
@@ -96,18 +113,40 @@ __global__ __launch_bounds__(1024) void bfs_vertex_push_kernel(
 {
     FrontierCount<R> fc;
     const int32_t nd = depth + 1;
+    const uint32_t lane = threadIdx.x & (kWave - 1);
     const uint64_t stride = static_cast<uint64_t>(blockDim.x) * gridDim.x;
-    for (uint64_t v = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; v < V; v += stride) {
-        if (levels[v] != depth) continue;
-        const uint64_t s = off[v], e = off[v + 1];
-        for (uint64_t j = s; j < e; j++)
-            if (relax_depth(levels, cols[j], nd)) fc.update(frontier);
+    // The loop runs while any lane of the wave has a vertex, so the wave
+    // stays converged for its ballots.
+    const uint64_t first = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const uint64_t waveFirst = first - lane;
+    for (uint64_t base = waveFirst; base < V; base += stride) {
+        const uint64_t v = base + lane;
+        const bool active = v < V && levels[v] == depth;
+        uint64_t s = 0, e = 0;
+        if (active) {
+            s = load_off(off, v);
+            e = load_off(off, v + 1);
+        }
+        const bool big = active && e - s > kWaveDegree;
+        for (uint64_t m = __ballot(big); m; m &= m - 1) {
+            const int l = __builtin_ctzll(m);
+            const uint64_t hs = read_lane_u64(s, l), he = read_lane_u64(e, l);
+            for (uint64_t j = hs + lane; j < he; j += kWave)
+                if (relax_depth(levels, cols[j], nd)) fc.update(frontier);
+        }
+        if (active && !big)
+            for (uint64_t j = s; j < e; j++)
+                if (relax_depth(levels, cols[j], nd)) fc.update(frontier);
     }
     fc.finalise(frontier);
 }
 
-// bfs/vertex_pull.cu: thread per unvisited vertex, first in-neighbour at
-// depth d wins (plain store, only this thread writes levels[v]).
+// bfs/vertex_pull.cu: a thread per unvisited vertex scans its in-edges
+// and the first in-neighbour at depth d sets it to d+1 (a plain store:
+// only this vertex's thread or wave writes levels[v]).  A vertex with more
+// than kWaveDegree in-edges is probed by the whole wave, 64 in-edges per
+// step, stopping at the first step where any lane finds one (a ballot);
+// which in-neighbour is found does not change the depth or the count.
 template<int R, typename Off>
 __global__ __launch_bounds__(1024) void bfs_vertex_pull_kernel(
     uint64_t V, const Off *__restrict__ roff, const uint32_t *__restrict__ rcols,
@@ -115,15 +154,38 @@ __global__ __launch_bounds__(1024) void bfs_vertex_pull_kernel(
 {
     FrontierCount<R> fc;
     const int32_t nd = depth + 1;
+    const uint32_t lane = threadIdx.x & (kWave - 1);
     const uint64_t stride = static_cast<uint64_t>(blockDim.x) * gridDim.x;
-    for (uint64_t v = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; v < V; v += stride) {
-        if (levels[v] <= nd) continue;
-        const uint64_t s = roff[v], e = roff[v + 1];
-        for (uint64_t j = s; j < e; j++) {
-            if (levels[rcols[j]] == depth) {
+    const uint64_t waveFirst = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x - lane;
+    for (uint64_t base = waveFirst; base < V; base += stride) {
+        const uint64_t v = base + lane;
+        const bool open = v < V && levels[v] > nd;
+        uint64_t s = 0, e = 0;
+        if (open) {
+            s = load_off(roff, v);
+            e = load_off(roff, v + 1);
+        }
+        const bool big = open && e - s > kWaveDegree;
+        for (uint64_t m = __ballot(big); m; m &= m - 1) {
+            const int l = __builtin_ctzll(m);
+            const uint64_t hs = read_lane_u64(s, l), he = read_lane_u64(e, l);
+            bool found = false;
+            for (uint64_t j0 = hs; j0 < he && !found; j0 += kWave) {
+                const uint64_t j = j0 + lane;
+                found = __ballot(j < he && levels[rcols[j]] == depth) != 0;
+            }
+            if (found && lane == static_cast<uint32_t>(l)) {
                 levels[v] = nd;
                 fc.update(frontier);
-                break;
+            }
+        }
+        if (open && !big) {
+            for (uint64_t j = s; j < e; j++) {
+                if (levels[rcols[j]] == depth) {
+                    levels[v] = nd;
+                    fc.update(frontier);
+                    break;
+                }
             }
         }
     }

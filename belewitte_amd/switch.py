@@ -238,13 +238,15 @@ def simulate(samples: Samples, choices: np.ndarray) -> dict:
 
 
 def evaluate(engine: BFSEngine, g, sources, model: Model, repeats: int = 1, runs: int = 3,
-             levels_per_sync: int = 8, samples_out=None) -> dict:
+             levels_per_sync: int = 8, samples_out=None, reference=None) -> dict:
     """Learned switch vs best fixed vs per-level oracle on `sources`.
 
     Simulated: sums of exhaustive per-level timings (device time of each
     level kernel).  Executed: whole searches run by the engine with the
     level loop on the device (hipGraph batches), min over `runs`; the
-    learned policy evaluates the model on the device at every level."""
+    learned policy evaluates the model on the device at every level.
+    `reference` ({source: depths} from an independent BFS) checks the
+    depths of the exhaustive and the learned runs of every source."""
     samples, results = collect(engine, sources, repeats=repeats)
     if samples_out:
         samples.save(samples_out)
@@ -264,8 +266,11 @@ def evaluate(engine: BFSEngine, g, sources, model: Model, repeats: int = 1, runs
     exec_learned = 0.0
     edges = 0
     mismatches = 0
+    ref_mismatches = 0
     for s in sources:
         levels_ref, counts_ref = results[int(s)]
+        if reference is not None and not np.array_equal(levels_ref, reference[int(s)]):
+            ref_mismatches += 1
         edges += teps_edges(g, levels_ref)
         for k in range(K):
             # A variant far off the best in the simulation is run once only.
@@ -283,6 +288,8 @@ def evaluate(engine: BFSEngine, g, sources, model: Model, repeats: int = 1, runs
         lv = engine.levels()
         if not (np.array_equal(lv, levels_ref) and np.array_equal(r.counts, counts_ref)):
             mismatches += 1
+        if reference is not None and not np.array_equal(lv, reference[int(s)]):
+            ref_mismatches += 1
     exec_best = int(np.argmin(exec_fixed))
     gteps = lambda ms: edges / (ms * 1e-3) / 1e9 if ms > 0 else 0.0  # noqa: E731
     return {
@@ -298,7 +305,8 @@ def evaluate(engine: BFSEngine, g, sources, model: Model, repeats: int = 1, runs
                         "learned": float(exec_learned), "fixed": dict(zip(VARIANTS, exec_fixed.tolist()))},
         "executed_gteps": {"learned": gteps(exec_learned), "best_fixed": gteps(exec_fixed[exec_best])},
         "executed_speedup_vs_best_fixed": float(exec_fixed[exec_best] / exec_learned) if exec_learned else None,
-        "depth_mismatches": mismatches,
+        "depth_mismatches_learned_vs_exhaustive": mismatches,
+        "depth_mismatches_vs_reference": ref_mismatches if reference is not None else None,
         "learned_choice_histogram": np.bincount(learned_choice, minlength=K).tolist(),
     }
 

@@ -226,11 +226,8 @@ int64_t oracle_bfs_vertex_push_omp(const oracle_graph *g, uint32_t root,
     if (seconds) *seconds = 0.0;
     if (root >= g->V) return 0;
     int64_t V = (int64_t)g->V;
-#ifdef _OPENMP
+    const int saved_threads = omp_get_max_threads();
     if (threads > 0) omp_set_num_threads(threads);
-#else
-    (void)threads;
-#endif
     /* The "computation" scope of bfs/bfs.cpp:78-108: initialisation and
      * the level loop. */
     const double t0 = now_seconds();
@@ -240,7 +237,10 @@ int64_t oracle_bfs_vertex_push_omp(const oracle_graph *g, uint32_t root,
     int64_t curr = 0;
     uint32_t frontier;
     do {
-        if (curr >= max_levels) return -1;
+        if (curr >= max_levels) {
+            omp_set_num_threads(saved_threads);
+            return -1;
+        }
         int32_t depth = (int32_t)curr, nd = depth + 1;
         uint32_t count = 0;
 #pragma omp parallel for schedule(dynamic, 1024) reduction(+ : count)
@@ -261,6 +261,7 @@ int64_t oracle_bfs_vertex_push_omp(const oracle_graph *g, uint32_t root,
         curr++;
     } while (frontier);
     if (seconds) *seconds = now_seconds() - t0;
+    omp_set_num_threads(saved_threads);
     return curr;
 }
 
@@ -477,11 +478,11 @@ void oracle_pagerank_pull_omp(const oracle_graph *g, int iterations, float *pr,
                               int threads, double *seconds)
 {
     int64_t V = (int64_t)g->V;
-#ifdef _OPENMP
+    /* The caller's thread count is restored on return, so a one-thread
+     * baseline does not leave later parallel regions of the process on
+     * one thread. */
+    const int saved_threads = omp_get_max_threads();
     if (threads > 0) omp_set_num_threads(threads);
-#else
-    (void)threads;
-#endif
     uint32_t *deg = (uint32_t *)malloc(sizeof(uint32_t) * (V ? V : 1));
     float *np = (float *)malloc(sizeof(float) * (V ? V : 1));
     /* The "computation" scope of pagerank/pagerank.cpp:95-127: degrees and
@@ -514,6 +515,7 @@ void oracle_pagerank_pull_omp(const oracle_graph *g, int iterations, float *pr,
     if (seconds) *seconds = now_seconds() - t0;
     free(deg);
     free(np);
+    omp_set_num_threads(saved_threads);
 }
 
 typedef struct {

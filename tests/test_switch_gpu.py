@@ -33,14 +33,16 @@ def test_learned_switch_beats_best_fixed(gpu, model):
     dg = device.DeviceGraph(g)
     eng = bfs_native.BFSEngine(dg)
     src = graph.random_sources(g, 4, seed=3)
-    res = switch.evaluate(eng, g, src, model, repeats=2, runs=3)
+    csr = O.CSR.from_graph(g)
+    ref = {int(s): O.bfs_queue(csr, int(s))[0] for s in src}
+    res = switch.evaluate(eng, g, src, model, repeats=2, runs=3, reference=ref)
     print(res["simulated_ms"], res["executed_ms"], flush=True)
-    assert res["depth_mismatches"] == 0
+    assert res["depth_mismatches_learned_vs_exhaustive"] == 0
+    assert res["depth_mismatches_vs_reference"] == 0
     assert res["simulated_speedup_vs_best_fixed"] >= 1.5, res["simulated_ms"]
     assert res["simulated_regret_vs_oracle"] <= 0.25, res["simulated_ms"]
     assert res["executed_speedup_vs_best_fixed"] >= 1.3, res["executed_ms"]
     # the learned runs themselves: exact depths from every source
-    csr = O.CSR.from_graph(g)
     eng.set_model(model.w)
     for s in src:
         r = eng.run(int(s), "learned", "td")
