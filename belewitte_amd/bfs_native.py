@@ -3,10 +3,12 @@
 Computes exactly the depths and per-level frontier counts of the
 reference's BFS (bfs/bfs.cpp:43-118 of merijn/Belewitte): INT_MAX for
 unreached vertices, one entry per level with the number of newly
-discovered vertices, the last entry 0.  Each level runs one of the native
-variants (``td``, ``bu``, ``bu-wave``), picked per level on the device by
-a fixed choice, a schedule, the on-device MLP or the host; because every
-variant yields the same depths, the choice never changes the result.
+discovered vertices, the last entry 0.  Each level runs one of the engine's
+variants -- the native ``td``, ``bu``, ``bu-wave`` and the reference's
+eight kernel families (bfs/*.cu) on the engine's state -- picked per level
+on the device by a fixed choice, a schedule, the on-device MLP, the host or
+Beamer's direction-optimising rule; because every variant yields the same
+depths, the choice never changes the result.
 """
 from __future__ import annotations
 
@@ -18,8 +20,16 @@ import numpy as np
 from . import _native as N
 from .device import DeviceGraph
 
-VARIANTS = ("td", "bu", "bu-wave")
-POLICIES = {"fixed": 0, "schedule": 1, "learned": 2, "host": 3}
+NATIVE_VARIANTS = ("td", "bu", "bu-wave")
+# The reference's families (bfs.KERNELS order, bfs/bfs.cpp:127-204) follow
+# the native variants.
+REFERENCE_VARIANTS = ("edge-list", "rev-edge-list", "struct-edge-list", "rev-struct-edge-list",
+                      "vertex-push", "vertex-pull", "vertex-push-warp", "vertex-pull-warp")
+VARIANTS = NATIVE_VARIANTS + REFERENCE_VARIANTS
+# Representation each reference family reads beyond both CSR directions.
+EDGE_LIST_REPRESENTATION = {"edge-list": "edge_list", "rev-edge-list": "rev_edge_list",
+                            "struct-edge-list": "struct_edge_list", "rev-struct-edge-list": "rev_struct_edge_list"}
+POLICIES = {"fixed": 0, "schedule": 1, "learned": 2, "host": 3, "beamer": 4}
 NUM_FEATURES = 8
 FEATURE_NAMES = ("log2_frontier_fraction", "log2_frontier_edge_fraction", "log2_unvisited_fraction", "depth",
                  "log2_mean_degree", "log2_std_degree", "prev_variant", "log2_prev_time_ns")
@@ -68,14 +78,24 @@ class LevelRecord:
 
 class BFSEngine:
     """Holds the engine's device state for one graph (both CSR directions,
-    uint32 offsets).  Repeated searches allocate nothing."""
+    uint32 offsets).  Repeated searches allocate nothing.
 
-    def __init__(self, dg: DeviceGraph, max_levels: int | None = None):
+    The edge-list families run when their representation is resident when
+    the engine is made: edge_lists=True uploads all four; otherwise those
+    the DeviceGraph already holds count."""
+
+    def __init__(self, dg: DeviceGraph, max_levels: int | None = None, edge_lists: bool = False):
         if dg.offset_bytes != 4:
             raise ValueError("the BFS engine reads uint32 offsets")
         dg.ensure("csr")
         dg.ensure("rev_csr")
+        if edge_lists:
+            for rep in EDGE_LIST_REPRESENTATION.values():
+                dg.ensure(rep)
         self.dg = dg
+        self.available = tuple(v for v in VARIANTS
+                               if v not in EDGE_LIST_REPRESENTATION or dg.edge_count == 0
+                               or dg.has(EDGE_LIST_REPRESENTATION[v]))
         V = dg.vertex_count
         self.max_levels = int(max_levels or min(V + 1, 1 << 22))
         p = C.c_void_p()
@@ -90,9 +110,13 @@ class BFSEngine:
     # ------------------------------------------------------------ set-up
 
     def set_model(self, weights: np.ndarray) -> None:
+        """A packed model with one output per native variant or per engine
+        variant (its size tells which)."""
         w = np.ascontiguousarray(weights, dtype=np.float32)
-        k = len(VARIANTS)
-        if w.size != int(N.hip_lib().bw_bfs_engine_mlp_len(k)):
+        lib = N.hip_lib()
+        k = next((k for k in (len(VARIANTS), len(NATIVE_VARIANTS)) if w.size == int(lib.bw_bfs_engine_mlp_len(k))),
+                 None)
+        if k is None:
             raise ValueError("model size does not match the engine's variant count")
         N.check_hip(N.hip_lib().bw_bfs_engine_set_model(self._e, w.ctypes.data_as(N.f32p), k))
         self.model_outputs = k
@@ -150,14 +174,19 @@ class BFSEngine:
         return {"done": bool(done.value), "depth": int(depth.value), "num_levels": int(nl.value),
                 "choice": int(choice.value), "features": feat}
 
-    def exhaustive(self, root: int, repeats: int = 1, stream=None):
-        """Times every variant at every level from the same saved state.
+    def exhaustive(self, root: int, repeats: int = 1, stream=None, variants=None):
+        """Times every available variant (or those named) at every level
+        from the same saved state; the others' times are inf.
 
         The search continues along the per-level fastest variant, so the
         features "previous variant" and "previous time" are those of the
         per-level oracle.  Returns (levels, counts, [LevelRecord])."""
         lib = N.hip_lib()
         K = len(VARIANTS)
+        run = [variant_id(v) for v in (variants if variants is not None else self.available)]
+        for k in run:
+            if VARIANTS[k] not in self.available:
+                raise ValueError(f"variant {VARIANTS[k]} needs a representation the engine was not given")
         N.check_hip(lib.bw_bfs_engine_begin(self._e, int(root), POLICIES["host"], 0, 1, stream))
         records, counts = [], []
         ms, cnt, done = C.c_float(), C.c_uint32(), C.c_uint32()
@@ -171,7 +200,7 @@ class BFSEngine:
             times = np.full(K, np.inf)
             seen = set()
             for r in range(repeats):
-                for k in range(K):
+                for k in run:
                     N.check_hip(lib.bw_bfs_engine_snapshot(self._e, 1, stream))
                     N.check_hip(lib.bw_bfs_engine_step_forced(self._e, k, C.byref(ms), C.byref(cnt),
                                                               C.byref(done), stream))

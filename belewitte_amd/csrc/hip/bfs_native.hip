@@ -47,11 +47,12 @@ constexpr uint32_t kNoParent = 0xFFFFFFFFu;
 constexpr int kFeatures = BW_BFS_FEATURES;
 constexpr int kHidden = 32;
 constexpr int kMaxOut = 16;
-constexpr int kVariants = BW_BFS_NATIVE_VARIANTS;
+constexpr int kVariants = BW_BFS_ENGINE_VARIANTS;  // native, then the reference families
+constexpr int kRef = BW_BFS_ENGINE_REFERENCE;
 
 enum Variant : int32_t { TD = 0, BU = 1, BU_WAVE = 2 };
 enum Policy : int32_t { FIXED = BW_POLICY_FIXED, SCHEDULE = BW_POLICY_SCHEDULE, LEARNED = BW_POLICY_LEARNED,
-                        HOST = BW_POLICY_HOST };
+                        HOST = BW_POLICY_HOST, BEAMER = BW_POLICY_BEAMER };
 
 struct BfsState {
     uint32_t depth;     // level being processed: the frontier is at `depth`
@@ -70,6 +71,8 @@ struct BfsState {
     uint64_t t_step;            // wall clock at the start of the level's step kernel
     uint32_t fill_levels;       // the previous level left its depths to this one (dense top-down)
     uint32_t pad_;
+    uint64_t explored_edges;    // out-degree sum of every vertex discovered so far
+    uint64_t prev_frontier;     // frontier size of the previous level (0 at level 0)
 };
 
 struct Graph {
@@ -79,7 +82,25 @@ struct Graph {
     const uint32_t *col;
     const uint32_t *roff;  // in-edges (bottom-up)
     const uint32_t *rcol;
+    // The edge-list representations the reference's edge-list families
+    // read (NULL when not resident): SoA forward (source, destination) and
+    // reverse (destination, source), AoS pairs of each.
+    const uint32_t *el_in, *el_out, *rel_in, *rel_out, *sel, *rsel;
 };
+
+// Whether variant c can run on this graph (the edge-list families need
+// their representation).
+__host__ __device__ inline bool variant_available(const Graph &g, int32_t c)
+{
+    if (c < 0 || c >= kVariants) return false;
+    switch (c - kRef) {
+    case BW_BFS_EDGE_LIST: return g.E == 0 || (g.el_in && g.el_out);
+    case BW_BFS_REV_EDGE_LIST: return g.E == 0 || (g.rel_in && g.rel_out);
+    case BW_BFS_STRUCT_EDGE_LIST: return g.E == 0 || g.sel;
+    case BW_BFS_REV_STRUCT_EDGE_LIST: return g.E == 0 || g.rsel;
+    default: return true;
+    }
+}
 
 struct Buffers {
     int32_t *levels;
@@ -281,7 +302,8 @@ __device__ void level_td(const Graph &g, const Buffers &b, uint32_t d, Emitter &
     }
 }
 
-__device__ void level_bu(const Graph &g, const Buffers &b, uint32_t d, Emitter &em, bool wave, bool fill)
+__device__ void level_bu(const Graph &g, const Buffers &b, uint32_t d, Emitter &em, bool wave, bool fill,
+                         bool use_parent = true)
 {
     const int32_t nd = static_cast<int32_t>(d) + 1;
     const uint32_t *front = b.bits[d % 3];
@@ -297,7 +319,7 @@ __device__ void level_bu(const Graph &g, const Buffers &b, uint32_t d, Emitter &
         const bool unvisited = v64 < V && !test_bit(b.vis, v);
         // The likely parent first: one coalesced load and one probe resolve
         // most vertices of a big level; the in-edges are scanned otherwise.
-        const uint32_t pv = unvisited ? b.parent[v] : kNoParent;
+        const uint32_t pv = unvisited && use_parent ? b.parent[v] : kNoParent;
         const bool quick = pv != kNoParent && test_bit(front, pv);
         uint32_t s = 0, e = 0;
         if (unvisited && !quick) {
@@ -359,6 +381,138 @@ __device__ void level_bu(const Graph &g, const Buffers &b, uint32_t d, Emitter &
     }
 }
 
+// ------------------------------------------ the reference's families
+
+// The reference's kernel families as engine variants (bfs/edgelist.cu,
+// rev_edgelist.cu, struct_edgelist.cu, rev_struct_edgelist.cu,
+// vertex_push.cu, vertex_pull.cu, vertex_push_warp.cu, vertex_pull_warp.cu):
+// each thread does the work the reference's thread does, with the frontier
+// test on the engine's level bitmap and discovery through its visited
+// bitmap and queues (so the next level of any variant can follow).
+
+// Edge-parallel scan of every stored edge: an edge whose tested endpoint is
+// in the frontier relaxes its other endpoint.  Forward lists test the
+// source, reverse lists (destination, source) the source as well.
+template<int FAMILY>
+__device__ void level_ref_edges(const Graph &g, const Buffers &b, uint32_t d, Emitter &em)
+{
+    const int32_t nd = static_cast<int32_t>(d) + 1;
+    const uint32_t *front = b.bits[d % 3];
+    const uint64_t E = g.E;
+    for (uint64_t base = static_cast<uint64_t>(blockIdx.x) * kBlock * 4; base < E;
+         base += static_cast<uint64_t>(gridDim.x) * kBlock * 4) {
+#pragma unroll
+        for (uint32_t k = 0; k < 4; k++) {
+            const uint64_t e = base + k * kBlock + threadIdx.x;
+            if (e >= E) continue;
+            uint32_t test, relax;
+            if constexpr (FAMILY == BW_BFS_EDGE_LIST) {
+                test = g.el_in[e];
+                relax = g.el_out[e];
+            } else if constexpr (FAMILY == BW_BFS_REV_EDGE_LIST) {
+                test = g.rel_out[e];
+                relax = g.rel_in[e];
+            } else if constexpr (FAMILY == BW_BFS_STRUCT_EDGE_LIST) {
+                test = g.sel[2 * e];
+                relax = g.sel[2 * e + 1];
+            } else {
+                test = g.rsel[2 * e + 1];
+                relax = g.rsel[2 * e];
+            }
+            if (test_bit(front, test) && try_discover(b, relax, nd, true)) em.emit(g, b, d, relax, true);
+        }
+        em.flush(b, d, 4 * kBlock);
+    }
+}
+
+// vertex_push.cu: a thread per vertex expands its out-edges when it is in
+// the frontier.
+__device__ void level_ref_push(const Graph &g, const Buffers &b, uint32_t d, Emitter &em)
+{
+    const int32_t nd = static_cast<int32_t>(d) + 1;
+    const uint32_t *front = b.bits[d % 3];
+    for (uint64_t base = static_cast<uint64_t>(blockIdx.x) * kBlock; base < g.V;
+         base += static_cast<uint64_t>(gridDim.x) * kBlock) {
+        const uint64_t v = base + threadIdx.x;
+        if (v < g.V && test_bit(front, static_cast<uint32_t>(v))) {
+            for (uint32_t j = g.off[v]; j < g.off[v + 1]; j++) {
+                const uint32_t w = g.col[j];
+                if (try_discover(b, w, nd, true)) em.emit(g, b, d, w, true);
+            }
+        }
+        em.flush(b, d, kBlock);
+    }
+}
+
+// vertex_push_warp.cu / vertex_pull_warp.cu geometry: a virtual warp of 32
+// lanes (half a wave) takes a chunk of 32 consecutive vertices and visits
+// them one after another.
+constexpr uint32_t kRefWarp = 32;
+
+__device__ void level_ref_push_warp(const Graph &g, const Buffers &b, uint32_t d, Emitter &em)
+{
+    const int32_t nd = static_cast<int32_t>(d) + 1;
+    const uint32_t *front = b.bits[d % 3];
+    const uint32_t lane = threadIdx.x % kRefWarp;
+    for (uint64_t base = static_cast<uint64_t>(blockIdx.x) * kBlock; base < g.V;
+         base += static_cast<uint64_t>(gridDim.x) * kBlock) {
+        const uint64_t cb = base + (threadIdx.x & ~(kRefWarp - 1));
+        for (uint32_t i = 0; i < kRefWarp && cb + i < g.V; i++) {
+            const uint32_t v = static_cast<uint32_t>(cb + i);
+            if (!test_bit(front, v)) continue;
+            for (uint32_t j = g.off[v] + lane; j < g.off[v + 1]; j += kRefWarp) {
+                const uint32_t w = g.col[j];
+                if (try_discover(b, w, nd, true)) em.emit(g, b, d, w, true);
+            }
+        }
+        em.flush(b, d, kBlock);
+    }
+}
+
+// vertex_pull_warp.cu: a virtual warp probes each unvisited vertex of its
+// chunk, its lanes striding over the in-edges and stopping at the first
+// stride with a hit (a ballot over the half wave).  A wave's two virtual
+// warps cover 64 consecutive vertices, two whole bitmap words, which it
+// writes with plain stores.
+__device__ void level_ref_pull_warp(const Graph &g, const Buffers &b, uint32_t d, Emitter &em)
+{
+    const int32_t nd = static_cast<int32_t>(d) + 1;
+    const uint32_t *front = b.bits[d % 3];
+    const uint32_t lane = threadIdx.x % kRefWarp;
+    const uint64_t half = (threadIdx.x & kRefWarp) ? ~0ull << 32 : 0xFFFFFFFFull;
+    for (uint64_t base = static_cast<uint64_t>(blockIdx.x) * kBlock; base < g.V;
+         base += static_cast<uint64_t>(gridDim.x) * kBlock) {
+        // The wave's 64 vertices, its half's 32 of them.
+        const uint64_t c0 = base + (threadIdx.x & ~(kRefWarp - 1));
+        uint32_t found_mask = 0;
+        for (uint32_t i = 0; i < kRefWarp; i++) {
+            const uint64_t v = c0 + i;
+            const bool open = v < g.V && !test_bit(b.vis, static_cast<uint32_t>(v));
+            bool found = false;
+            if (open) {
+                const uint32_t s = g.roff[v], e = g.roff[v + 1];
+                for (uint32_t j0 = s; j0 < e && !found; j0 += kRefWarp) {
+                    const uint32_t j = j0 + lane;
+                    found = (__ballot(j < e && test_bit(front, g.rcol[j])) & half) != 0;
+                }
+            }
+            if (found) {
+                found_mask |= 1u << i;
+                if (lane == 0) {
+                    b.levels[v] = nd;
+                    em.emit(g, b, d, static_cast<uint32_t>(v), false);
+                }
+            }
+        }
+        if (lane == 0 && found_mask) {
+            const uint64_t wi = c0 >> 5;
+            b.bits[(d + 1) % 3][wi] = found_mask;
+            b.vis[wi] |= found_mask;
+        }
+        em.flush(b, d, kBlock);
+    }
+}
+
 // ---------------------------------------------------------- level kernel
 
 __global__ __launch_bounds__(kBlock) void bfs_engine_step(Graph g, Buffers b)
@@ -385,16 +539,26 @@ __global__ __launch_bounds__(kBlock) void bfs_engine_step(Graph g, Buffers b)
     __syncthreads();
     Emitter em{lbuf, &lcount, &lbase};
     const bool fill = st.fill_levels != 0;
-    if (variant == TD) {
+    if (variant == BU || variant == BU_WAVE || variant == kRef + BW_BFS_VERTEX_PULL) {
+        // Bottom-up forms write the deferred depths in their vertex loop.
+        level_bu(g, b, d, em, variant == BU_WAVE, fill, variant != kRef + BW_BFS_VERTEX_PULL);
+    } else {
         if (fill) {  // the previous level's depths, from this level's frontier bitmap
             const uint32_t *front = b.bits[d % 3];
             for (uint64_t v = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x; v < g.V;
                  v += static_cast<uint64_t>(gridDim.x) * kBlock)
                 if (test_bit(front, static_cast<uint32_t>(v))) b.levels[v] = static_cast<int32_t>(d);
         }
-        level_td(g, b, d, em, sstart, sscan, &stotal);
-    } else {
-        level_bu(g, b, d, em, variant == BU_WAVE, fill);
+        switch (variant - kRef) {
+        case BW_BFS_EDGE_LIST: level_ref_edges<BW_BFS_EDGE_LIST>(g, b, d, em); break;
+        case BW_BFS_REV_EDGE_LIST: level_ref_edges<BW_BFS_REV_EDGE_LIST>(g, b, d, em); break;
+        case BW_BFS_STRUCT_EDGE_LIST: level_ref_edges<BW_BFS_STRUCT_EDGE_LIST>(g, b, d, em); break;
+        case BW_BFS_REV_STRUCT_EDGE_LIST: level_ref_edges<BW_BFS_REV_STRUCT_EDGE_LIST>(g, b, d, em); break;
+        case BW_BFS_VERTEX_PUSH: level_ref_push(g, b, d, em); break;
+        case BW_BFS_VERTEX_PUSH_WARP: level_ref_push_warp(g, b, d, em); break;
+        case BW_BFS_VERTEX_PULL_WARP: level_ref_pull_warp(g, b, d, em); break;
+        default: level_td(g, b, d, em, sstart, sscan, &stotal);
+        }
     }
     em.flush(b, d, kLdsQueue);  // reserve everything: moves all that is staged
 
@@ -452,9 +616,11 @@ __host__ __device__ void features(const BfsState &st, uint64_t V, uint64_t E, fl
 }
 
 // One workgroup: MLP forward on the features (one output per variant, the
-// predicted log2 time), returns the argmin.  `w` is the workgroup's LDS
-// copy of the packed weights (see stage_model).
-__device__ int32_t mlp_choose(const float *w, int K, const float *f, float *h1, float *h2, float *out)
+// predicted log2 time), returns the argmin over the variants this graph can
+// run (-1 if none).  `w` is the workgroup's LDS copy of the packed weights
+// (see stage_model).
+__device__ int32_t mlp_choose(const Graph &g, const float *w, int K, const float *f, float *h1, float *h2,
+                              float *out)
 {
     const float *mean = w, *sd = w + kFeatures;
     const float *W1 = sd + kFeatures, *b1 = W1 + kHidden * kFeatures;
@@ -479,9 +645,9 @@ __device__ int32_t mlp_choose(const float *w, int K, const float *f, float *h1, 
         out[t] = a;
     }
     __syncthreads();
-    int32_t best = 0;
-    for (int k = 1; k < K; k++)
-        if (out[k] < out[best]) best = k;
+    int32_t best = -1;
+    for (int k = 0; k < K; k++)
+        if (variant_available(g, k) && (best < 0 || out[k] < out[best])) best = k;
     return best;
 }
 
@@ -504,10 +670,23 @@ __device__ int32_t choose(const PolicyArgs &p, const BfsState &st, const Graph &
     case LEARNED: {
         if (threadIdx.x == 0) features(st, g.V, g.E, f);
         __syncthreads();
-        const int32_t c = mlp_choose(sw, p.nout, f, h1, h2, out);
-        return c < kVariants ? c : p.fixed;
+        const int32_t c = mlp_choose(g, sw, p.nout, f, h1, h2, out);
+        return c >= 0 ? c : p.fixed;
     }
     case HOST: return st.choice;
+    case BEAMER: {
+        // After a top-down level: bottom-up once the frontier's out-edges
+        // m_f exceed the unexplored vertices' out-edges m_u / 14.  After a
+        // bottom-up level: top-down again once the frontier both shrinks
+        // and holds fewer than V / 24 vertices.
+        const double mf = static_cast<double>(st.frontier_edges);
+        const double mu = static_cast<double>(g.E) - static_cast<double>(st.explored_edges);
+        const uint64_t nf = static_cast<uint64_t>(st.qcount) + st.hubcount;
+        const bool bottom_up = st.prev_choice == BU_WAVE || st.prev_choice == BU;
+        if (!bottom_up) return mf > mu / 14.0 ? BU_WAVE : TD;
+        const bool stay = nf >= st.prev_frontier || static_cast<double>(nf) > static_cast<double>(g.V) / 24.0;
+        return stay ? BU_WAVE : TD;
+    }
     default: return p.fixed;
     }
 }
@@ -599,12 +778,14 @@ __global__ __launch_bounds__(kBlock) void bfs_engine_epilogue(Graph g, Buffers b
         } else {
             st.prev_choice = st.choice;
             st.prev_time_ns = ns;
+            st.prev_frontier = static_cast<uint64_t>(st.qcount) + st.hubcount;
             st.depth = d + 1;
             st.qcount = st.qout;
             st.hubcount = nhub;
             st.qout = 0;
             st.hubout = 0;
             st.visited += nnew;
+            st.explored_edges += a;
             st.frontier_edges = a;
             st.frontier_sq = q;
             st.fill_levels = deferred ? 1u : 0u;
@@ -656,6 +837,8 @@ __global__ __launch_bounds__(kBlock) void bfs_engine_init(Graph g, Buffers b, Po
         st.hubout = 0;
         st.num_levels = 0;
         st.visited = visited;
+        st.explored_edges = edges;
+        st.prev_frontier = 0;
         st.frontier_edges = edges;
         st.frontier_sq = sq;
         st.prev_choice = -1;
@@ -775,16 +958,19 @@ PolicyArgs make_policy(const bw_bfs_engine *e, int32_t policy, int32_t fixed, in
     return p;
 }
 
-int check_variant(int32_t v)
+int check_variant(const bw_bfs_engine *e, int32_t v)
 {
-    if (v < 0 || v >= kVariants) return fail(BW_HIP_ERR_ARG, "unknown native BFS variant " + std::to_string(v));
+    if (v < 0 || v >= kVariants) return fail(BW_HIP_ERR_ARG, "unknown BFS engine variant " + std::to_string(v));
+    if (!variant_available(e->g, v))
+        return fail(BW_HIP_ERR_ARG, "BFS engine variant " + std::to_string(v) +
+                                        " needs an edge-list representation the engine was not given");
     return BW_HIP_OK;
 }
 
 int check_policy(const bw_bfs_engine *e, int32_t policy, int32_t fixed, int32_t schedule_len)
 {
-    BW_TRY(check_variant(fixed));
-    if (policy < FIXED || policy > HOST) return fail(BW_HIP_ERR_ARG, "unknown policy " + std::to_string(policy));
+    BW_TRY(check_variant(e, fixed));
+    if (policy < FIXED || policy > BEAMER) return fail(BW_HIP_ERR_ARG, "unknown policy " + std::to_string(policy));
     if (policy == LEARNED && !e->mlp) return fail(BW_HIP_ERR_ARG, "no model set for the learned policy");
     if (policy == SCHEDULE && (!e->schedule || schedule_len < 1 || schedule_len > e->schedule_cap))
         return fail(BW_HIP_ERR_ARG, "no schedule of that length set");
@@ -816,8 +1002,9 @@ extern "C" int bw_bfs_engine_create(bw_bfs_engine **out, const bw_graph_dev *g, 
     if (rate_khz <= 0) return fail(BW_HIP_ERR_RUNTIME, "device reports no wall clock rate");
     auto e = new bw_bfs_engine();
     const uint64_t V = g->vertex_count;
-    e->g = Graph{V, g->edge_count, static_cast<const uint32_t *>(g->offsets), g->cols,
-                 static_cast<const uint32_t *>(g->rev_offsets), g->rev_cols};
+    e->g = Graph{V,       g->edge_count, static_cast<const uint32_t *>(g->offsets), g->cols,
+                 static_cast<const uint32_t *>(g->rev_offsets), g->rev_cols, g->el_in, g->el_out, g->rel_in,
+                 g->rel_out, g->sel, g->rsel};
     e->grid = static_cast<uint32_t>(device_cus()) * 8;
     const uint32_t words = static_cast<uint32_t>((V + 31) / 32);
     bool ok = true;
@@ -880,8 +1067,9 @@ extern "C" int32_t bw_bfs_engine_mlp_len(int32_t outputs) { return mlp_len(outpu
 
 extern "C" int bw_bfs_engine_set_model(bw_bfs_engine *e, const float *weights, int32_t outputs)
 {
-    if (!e || !weights || outputs != kVariants)
-        return fail(BW_HIP_ERR_ARG, "bw_bfs_engine_set_model: the model needs one output per native variant");
+    if (!e || !weights || (outputs != BW_BFS_NATIVE_VARIANTS && outputs != BW_BFS_ENGINE_VARIANTS))
+        return fail(BW_HIP_ERR_ARG, "bw_bfs_engine_set_model: the model needs one output per native variant, or "
+                                    "per engine variant");
     if (!e->mlp) BW_HIP_CHECK(hipMalloc(&e->mlp, sizeof(float) * mlp_len(kMaxOut)));
     BW_HIP_CHECK(hipMemcpy(e->mlp, weights, sizeof(float) * mlp_len(outputs), hipMemcpyHostToDevice));
     e->mlp_out = outputs;
@@ -891,7 +1079,7 @@ extern "C" int bw_bfs_engine_set_model(bw_bfs_engine *e, const float *weights, i
 extern "C" int bw_bfs_engine_set_schedule(bw_bfs_engine *e, const int32_t *schedule, int32_t len)
 {
     if (!e || !schedule || len < 1) return fail(BW_HIP_ERR_ARG, "invalid schedule");
-    for (int32_t i = 0; i < len; i++) BW_TRY(check_variant(schedule[i]));
+    for (int32_t i = 0; i < len; i++) BW_TRY(check_variant(e, schedule[i]));
     if (len > e->schedule_cap) {
         e->drop_exec();  // the captured batch holds the old schedule pointer
         if (e->schedule) (void)hipFree(e->schedule);
@@ -985,7 +1173,7 @@ extern "C" int bw_bfs_engine_level(bw_bfs_engine *e, int32_t policy, int32_t fix
 extern "C" int bw_bfs_engine_force(bw_bfs_engine *e, int32_t variant, void *stream)
 {
     if (!e) return fail(BW_HIP_ERR_ARG, "engine is NULL");
-    BW_TRY(check_variant(variant));
+    BW_TRY(check_variant(e, variant));
     hipLaunchKernelGGL(set_choice, dim3(1), dim3(1), 0, static_cast<hipStream_t>(stream), e->b.st, variant);
     BW_LAUNCH_CHECK();
     return BW_HIP_OK;

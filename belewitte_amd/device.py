@@ -201,6 +201,13 @@ class DeviceGraph:
             setattr(d, ko, a_out.ptr)
         synchronize()
 
+    _KEY = {"csr": "off", "rev_csr": "roff", "edge_list": "el_in", "rev_edge_list": "rel_in",
+            "struct_edge_list": "sel", "rev_struct_edge_list": "rsel"}
+
+    def has(self, rep: str) -> bool:
+        """Whether a representation is resident."""
+        return self._KEY[rep] in self._a
+
     def array(self, key: str) -> DeviceArray:
         return self._a[key]
 

@@ -6,9 +6,9 @@ and compares it with the best non-switching implementation and the
 per-level optimum (benchmark-analysis/src/StepAggregate.hs:95-119,
 benchmark-analysis/model-src/Evaluate.hs:229-230).  Here:
 
-* :func:`collect` times every native variant at every level from the same
-  saved state (exhaustive per-level timing) and records the level's
-  features;
+* :func:`collect` times every engine variant (the native ones and the
+  reference's eight kernel families) at every level from the same saved
+  state (exhaustive per-level timing) and records the level's features;
 * :class:`Model` is the FP32 MLP 8 -> 32 -> 32 -> K, trained by
   :meth:`Model.train` with the hand-written GPU kernels (bw_mlp_train),
   predicting the log2 level time of each variant;
@@ -16,10 +16,11 @@ benchmark-analysis/model-src/Evaluate.hs:229-230).  Here:
   levels of the fastest variant), the best fixed variant (the variant with
   the least total time) and the learned switch, both simulated on the
   exhaustive timings and executed end to end by the engine with the model
-  evaluated on the device.
+  evaluated on the device, beside Beamer's direction-optimising rule.
 """
 from __future__ import annotations
 
+import hashlib
 import json
 from dataclasses import dataclass
 from pathlib import Path
@@ -85,9 +86,10 @@ class Samples:
 class Model:
     """Packed FP32 weights of the MLP (mean, std, W1, b1, W2, b2, W3, b3)."""
 
-    def __init__(self, weights: np.ndarray, outputs: int = len(VARIANTS)):
+    def __init__(self, weights: np.ndarray, outputs: int = len(VARIANTS), provenance: dict | None = None):
         self.K = int(outputs)
         self.w = np.ascontiguousarray(weights, dtype=np.float32)
+        self.provenance = dict(provenance or {})
         if self.w.size != num_params(self.K):
             raise ValueError(f"expected {num_params(self.K)} parameters, got {self.w.size}")
 
@@ -160,14 +162,15 @@ class Model:
     def save(self, path) -> None:
         Path(path).write_text(json.dumps({"format": "belewitte-bfs-mlp", "outputs": self.K,
                                           "variants": list(VARIANTS[:self.K]), "hidden": HIDDEN,
-                                          "features": NUM_FEATURES, "weights": self.w.tolist()}))
+                                          "features": NUM_FEATURES, "provenance": self.provenance,
+                                          "weights": self.w.tolist()}))
 
     @classmethod
     def load(cls, path) -> "Model":
         d = json.loads(Path(path).read_text())
         if d.get("format") != "belewitte-bfs-mlp":
             raise ValueError(f"{path} is not a BFS switch model")
-        return cls(np.asarray(d["weights"], dtype=np.float32), int(d["outputs"]))
+        return cls(np.asarray(d["weights"], dtype=np.float32), int(d["outputs"]), d.get("provenance"))
 
 
 def collect(engine: BFSEngine, sources, repeats: int = 1, progress=None):
@@ -185,9 +188,13 @@ def collect(engine: BFSEngine, sources, repeats: int = 1, progress=None):
 
 
 def fit(samples: Samples, epochs: int = 100, batch: int = 64, lr: float = 3e-3, seed: int = 0):
-    """A model trained on the GPU on `samples`; returns (model, epoch losses)."""
+    """A model trained on the GPU on `samples`; returns (model, epoch losses).
+    Every variant must be timed at every level (no inf)."""
+    if not np.all(np.isfinite(samples.ms)):
+        raise ValueError("training needs every variant timed at every level")
     Y = samples.targets
-    m = Model.init(samples.X, seed=seed, target_mean=float(Y.mean()) if Y.size else 0.0)
+    m = Model.init(samples.X, outputs=samples.ms.shape[1], seed=seed,
+                   target_mean=float(Y.mean()) if Y.size else 0.0)
     loss = m.train(samples.X, Y, epochs=epochs, batch=batch, lr=lr, seed=seed + 1)
     return m, loss
 
@@ -238,21 +245,28 @@ def simulate(samples: Samples, choices: np.ndarray) -> dict:
 
 
 def evaluate(engine: BFSEngine, g, sources, model: Model, repeats: int = 1, runs: int = 3,
-             levels_per_sync: int = 8, samples_out=None, reference=None) -> dict:
-    """Learned switch vs best fixed vs per-level oracle on `sources`.
+             levels_per_sync: int = 8, samples_out=None, reference=None, slow_factor: float = 20.0) -> dict:
+    """Learned switch vs best fixed vs per-level oracle vs Beamer's rule on
+    `sources`, over every variant the engine can run.
 
     Simulated: sums of exhaustive per-level timings (device time of each
-    level kernel).  Executed: whole searches run by the engine with the
+    level kernel); Beamer's choices are those of its executed runs (every
+    variant reaches the same frontiers, so its levels align with the
+    exhaustive ones).  Executed: whole searches run by the engine with the
     level loop on the device (hipGraph batches), min over `runs`; the
-    learned policy evaluates the model on the device at every level.
+    learned policy evaluates the model on the device at every level.  A
+    fixed variant slower than `slow_factor` times the best in the
+    simulation is not executed (its executed time is reported as None).
     `reference` ({source: depths} from an independent BFS) checks the
-    depths of the exhaustive and the learned runs of every source."""
+    depths of the exhaustive, learned and Beamer runs of every source."""
     samples, results = collect(engine, sources, repeats=repeats)
     if samples_out:
         samples.save(samples_out)
-    learned_choice = model.choose(samples.X)
-    sim = simulate(samples, learned_choice)
     K = len(VARIANTS)
+    pred = model.predict(samples.X)
+    avail = np.array([v in engine.available for v in VARIANTS[:model.K]])
+    learned_choice = np.argmin(np.where(avail[None, :], pred, np.inf), axis=1)
+    sim = simulate(samples, learned_choice)
     fixed_tot = np.zeros(K)
     oracle_tot = switch_tot = 0.0
     for s in sim.values():
@@ -263,21 +277,22 @@ def evaluate(engine: BFSEngine, g, sources, model: Model, repeats: int = 1, runs
 
     engine.set_model(model.w)
     exec_fixed = np.zeros(K)
-    exec_learned = 0.0
+    run_fixed = np.isfinite(fixed_tot) & (fixed_tot <= slow_factor * fixed_tot[best_fixed])
+    exec_learned = exec_beamer = sim_beamer = 0.0
     edges = 0
-    mismatches = 0
-    ref_mismatches = 0
+    mismatches = ref_mismatches = beamer_mismatches = 0
+    beamer_hist = np.zeros(K, dtype=np.int64)
     for s in sources:
         levels_ref, counts_ref = results[int(s)]
         if reference is not None and not np.array_equal(levels_ref, reference[int(s)]):
             ref_mismatches += 1
         edges += teps_edges(g, levels_ref)
-        for k in range(K):
+        for k in np.flatnonzero(run_fixed):
             # A variant far off the best in the simulation is run once only.
             reps = runs if fixed_tot[k] <= 2.0 * fixed_tot[best_fixed] else 1
             best = np.inf
             for _ in range(reps):
-                r = engine.run(int(s), "fixed", k, levels_per_sync=levels_per_sync, download=False)
+                r = engine.run(int(s), "fixed", int(k), levels_per_sync=levels_per_sync, download=False)
                 best = min(best, r.total_ms)
             exec_fixed[k] += best
         best = np.inf
@@ -290,24 +305,45 @@ def evaluate(engine: BFSEngine, g, sources, model: Model, repeats: int = 1, runs
             mismatches += 1
         if reference is not None and not np.array_equal(lv, reference[int(s)]):
             ref_mismatches += 1
+        best = np.inf
+        for _ in range(runs):
+            rb = engine.run(int(s), "beamer", "td", levels_per_sync=levels_per_sync, download=False)
+            best = min(best, rb.total_ms)
+        exec_beamer += best
+        lv = engine.levels()
+        if not (np.array_equal(lv, levels_ref) and np.array_equal(rb.counts, counts_ref)):
+            beamer_mismatches += 1
+        m = samples.source == int(s)
+        t = samples.ms[m]
+        sim_beamer += float(t[np.arange(t.shape[0]), rb.choices[:t.shape[0]]].sum())
+        beamer_hist += np.bincount(rb.choices, minlength=K)
+    exec_fixed = np.where(run_fixed, exec_fixed, np.inf)
     exec_best = int(np.argmin(exec_fixed))
     gteps = lambda ms: edges / (ms * 1e-3) / 1e9 if ms > 0 else 0.0  # noqa: E731
+    fin = lambda x: float(x) if np.isfinite(x) else None  # noqa: E731
     return {
         "sources": [int(s) for s in sources],
         "levels": int(samples.X.shape[0]),
         "teps_edges": int(edges),
+        "variants": list(engine.available),
         "simulated_ms": {"oracle": oracle_tot, "best_fixed": float(fixed_tot[best_fixed]),
-                         "best_fixed_variant": VARIANTS[best_fixed], "learned": switch_tot,
-                         "fixed": dict(zip(VARIANTS, fixed_tot.tolist()))},
+                         "best_fixed_variant": VARIANTS[best_fixed], "learned": switch_tot, "beamer": sim_beamer,
+                         "fixed": {v: fin(x) for v, x in zip(VARIANTS, fixed_tot)}},
         "simulated_speedup_vs_best_fixed": float(fixed_tot[best_fixed] / switch_tot) if switch_tot else None,
         "simulated_regret_vs_oracle": float(switch_tot / oracle_tot - 1.0) if oracle_tot else None,
+        "simulated_beamer_regret_vs_oracle": float(sim_beamer / oracle_tot - 1.0) if oracle_tot else None,
         "executed_ms": {"best_fixed": float(exec_fixed[exec_best]), "best_fixed_variant": VARIANTS[exec_best],
-                        "learned": float(exec_learned), "fixed": dict(zip(VARIANTS, exec_fixed.tolist()))},
-        "executed_gteps": {"learned": gteps(exec_learned), "best_fixed": gteps(exec_fixed[exec_best])},
+                        "learned": float(exec_learned), "beamer": float(exec_beamer),
+                        "fixed": {v: fin(x) for v, x in zip(VARIANTS, exec_fixed)}},
+        "executed_gteps": {"learned": gteps(exec_learned), "best_fixed": gteps(exec_fixed[exec_best]),
+                           "beamer": gteps(exec_beamer)},
         "executed_speedup_vs_best_fixed": float(exec_fixed[exec_best] / exec_learned) if exec_learned else None,
+        "executed_speedup_vs_beamer": float(exec_beamer / exec_learned) if exec_learned else None,
         "depth_mismatches_learned_vs_exhaustive": mismatches,
+        "depth_mismatches_beamer_vs_exhaustive": beamer_mismatches,
         "depth_mismatches_vs_reference": ref_mismatches if reference is not None else None,
-        "learned_choice_histogram": np.bincount(learned_choice, minlength=K).tolist(),
+        "learned_choice_histogram": dict(zip(VARIANTS, np.bincount(learned_choice, minlength=K).tolist())),
+        "beamer_choice_histogram": dict(zip(VARIANTS, beamer_hist.tolist())),
     }
 
 
@@ -330,6 +366,20 @@ def training_graphs(small: bool = False):
     yield G.rmat(22, 16, seed=2)
 
 
+ENGINE_SOURCES = ("csrc/hip/bfs_native.hip", "csrc/hip/mlp.hip", "../include/belewitte_hip.h")
+
+
+def engine_fingerprint() -> dict:
+    """What identifies the engine a model was trained on: the library's
+    build version and a SHA-256 over the engine's sources."""
+    h = hashlib.sha256()
+    here = Path(__file__).resolve().parent
+    for rel in ENGINE_SOURCES:
+        h.update((here / rel).read_bytes())
+    return {"library": N.hip_lib().bw_hip_version().decode(), "sources_sha256": h.hexdigest(),
+            "sources": [str(Path(r).name) for r in ENGINE_SOURCES]}
+
+
 def main(argv=None) -> int:
     import argparse
     import sys
@@ -346,12 +396,12 @@ def main(argv=None) -> int:
     args = ap.parse_args(argv)
     device.require_gpu()
     device.set_device(0)
-    groups = []
+    groups, trained_on = [], []
     for g in training_graphs(args.small):
         t0 = time.perf_counter()
         print(f"[switch] {g.name} generated", file=sys.stderr, flush=True)
         dg = device.DeviceGraph(g)
-        eng = BFSEngine(dg)
+        eng = BFSEngine(dg, edge_lists=True)
         src = G.random_sources(g, args.sources, seed=5)
         s, _ = collect(eng, src, progress=lambda i, v, n: print(
             f"[switch] {g.name}: source {i + 1}/{len(src)} ({v}): {n} levels", file=sys.stderr, flush=True))
@@ -360,6 +410,8 @@ def main(argv=None) -> int:
               f"oracle {s.ms.min(axis=1).sum():.3f} ms", file=sys.stderr, flush=True)
         s.source = s.source + (np.int64(len(groups)) << 32)  # sources unique across graphs
         groups.append(s)
+        trained_on.append({"graph": g.name, "vertices": int(g.vertex_count), "edges": int(g.edge_count),
+                           "sources": len(src), "source_seed": 5, "levels": int(s.X.shape[0])})
         eng.close()
         dg.free()
     samples = groups[0]
@@ -368,6 +420,7 @@ def main(argv=None) -> int:
     if args.samples_out:
         samples.save(args.samples_out)
     epochs = args.epochs
+    cv = None
     if epochs <= 0:
         cv = cross_validate(groups)
         epochs = min(cv, key=cv.get)
@@ -382,6 +435,20 @@ def main(argv=None) -> int:
     print(f"[switch] trained: loss {loss[0]:.4f} -> {loss[-1]:.4f}; training-set totals: oracle {oracle:.3f} ms, "
           f"learned {learned:.3f} ms, best fixed {fixed.min():.3f} ms ({VARIANTS[int(np.argmin(fixed))]})",
           file=sys.stderr, flush=True)
+    import datetime
+    props = device.device_props(0)
+    model.provenance = {
+        "engine": engine_fingerprint(),
+        "device": f"{props['name']} ({props['arch']}, {props['compute_units']} CUs)",
+        "trained": datetime.datetime.now(datetime.timezone.utc).isoformat(timespec="seconds"),
+        "graphs": trained_on,
+        "epochs": int(epochs), "seed": 0,
+        "cross_validation_regret": {str(k): v for k, v in cv.items()} if cv else None,
+        "training_set_ms": {"oracle": float(oracle), "learned": float(learned),
+                            "fixed": dict(zip(VARIANTS, np.asarray(fixed, dtype=float).tolist()))},
+        "command": "python -m belewitte_amd.switch" + (" --small" if args.small else "")
+                   + f" --sources {args.sources}" + (f" --epochs {args.epochs}" if args.epochs else ""),
+    }
     Path(args.out).parent.mkdir(parents=True, exist_ok=True)
     model.save(args.out)
     print(args.out)

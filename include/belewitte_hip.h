@@ -528,11 +528,29 @@ typedef struct bw_bfs_engine bw_bfs_engine;
 #define BW_BFS_NATIVE_BU 1      /* bottom-up, thread per unvisited vertex   */
 #define BW_BFS_NATIVE_BU_WAVE 2 /* bottom-up, wave64 probes for big rows    */
 #define BW_BFS_NATIVE_VARIANTS 3
+/* Engine variants BW_BFS_ENGINE_REFERENCE + k run the reference's kernel
+ * family k (BW_BFS_EDGE_LIST .. BW_BFS_VERTEX_PULL_WARP of the bfs/ kernels): the
+ * same work per thread (edge-parallel scans of the stored edge lists,
+ * thread-per-vertex push and pull, 32-lane virtual warps over 32-vertex
+ * chunks), with the frontier read from the engine's level bitmap and
+ * discoveries going to its queue, so any variant can run any level of a
+ * device-driven search.  The edge-list families need the matching
+ * representations in the bw_graph_dev given to bw_bfs_engine_create. */
+#define BW_BFS_ENGINE_REFERENCE 3
+#define BW_BFS_ENGINE_VARIANTS 11
 
 #define BW_POLICY_FIXED 0
 #define BW_POLICY_SCHEDULE 1
 #define BW_POLICY_LEARNED 2
 #define BW_POLICY_HOST 3
+/* The direction-optimising heuristic of Beamer, Asanovic and Patterson
+ * (SC 2012) with alpha = 14, beta = 24.  After a top-down level (or at
+ * level 0): bottom-up (BU_WAVE) when the frontier's out-edges m_f exceed
+ * m_u / alpha, m_u = E minus the out-degrees of every vertex discovered so
+ * far (frontier included); else top-down.  After a bottom-up level:
+ * top-down when the frontier is smaller than the previous one and holds
+ * fewer than V / beta vertices; else bottom-up. */
+#define BW_POLICY_BEAMER 4
 
 /* Features of a level, reduced on the device: frontier vertices, frontier
  * out-edges and unvisited vertices as log2 fractions of the graph
@@ -565,7 +583,9 @@ int bw_bfs_engine_set_tuning(bw_bfs_engine *engine, uint32_t hub_degree,
 /* Float count of a packed model with `outputs` outputs: mean[8] std[8]
  * W1[32][8] b1[32] W2[32][32] b2[32] W3[K][32] b3[K] (row-major, ReLU
  * hidden layers, outputs = predicted log2 level time per variant; the
- * argmin is taken). */
+ * argmin is taken over the variants the engine can run, i.e. those whose
+ * representation its graph holds).  outputs is BW_BFS_NATIVE_VARIANTS or
+ * BW_BFS_ENGINE_VARIANTS. */
 int32_t bw_bfs_engine_mlp_len(int32_t outputs);
 int bw_bfs_engine_set_model(bw_bfs_engine *engine, const float *weights_host,
                             int32_t outputs);

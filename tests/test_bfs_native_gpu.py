@@ -4,8 +4,10 @@ Depths must equal the oracle's textbook queue BFS and the per-level counts
 the oracle's restatement of the reference level loop (bfs/bfs.cpp:93-107:
 newly discovered vertices per level, last 0), for every native variant,
 fixed and per-level schedules, host-stepped and hipGraph-batched loops,
-the learned policy with any model, hubs beyond the hub-queue capacity,
-unreachable vertices and high-diameter graphs.  The frontier features
+the learned policy with any model, Beamer's direction-optimising rule,
+hubs beyond the hub-queue capacity, unreachable vertices and
+high-diameter graphs.  The variants are the native ones and the
+reference's eight kernel families run on the engine's state.  The frontier features
 reduced on the device are checked against the same quantities computed on
 the host from the oracle's depths.
 """
@@ -66,7 +68,7 @@ def cases(gpu):
     for g in _graphs():
         csr = O.CSR.from_graph(g)
         dg = device.DeviceGraph(g)
-        eng = bfs_native.BFSEngine(dg)
+        eng = bfs_native.BFSEngine(dg, edge_lists=True)
         roots = sorted({g.max_out_degree_vertex(), 0, g.vertex_count // 2, g.vertex_count - 1})
         out.append((g, csr, dg, eng, roots))
     yield out
@@ -123,7 +125,7 @@ def test_many_hubs(gpu):
     g = _many_hubs()
     csr = O.CSR.from_graph(g)
     dg = device.DeviceGraph(g)
-    eng = bfs_native.BFSEngine(dg)
+    eng = bfs_native.BFSEngine(dg, edge_lists=True)
     try:
         want, counts = _want(csr, 0)
         for v in bfs_native.VARIANTS:
@@ -183,6 +185,88 @@ def test_device_choice_matches_host_model(cases):
             N.check_hip(lib.bw_bfs_engine_level(eng._e, bfs_native.POLICIES["learned"], 0, 1, None))
 
 
+def _beamer_choices(g, levels):
+    """Beamer's rule per level (alpha 14, beta 24), from the oracle's depths:
+    after top-down, bottom-up when m_f > m_u / 14 (m_u: E minus the
+    out-degrees of every vertex reached so far); after bottom-up, top-down
+    when the frontier shrank below V / 24."""
+    TD, BU = bfs_native.variant_id("td"), bfs_native.variant_id("bu-wave")
+    deg = np.diff(g.offsets()).astype(np.int64)
+    reached = levels != bfs_native.UNVISITED
+    depth = levels[reached]
+    nd = int(depth.max()) + 1
+    nf = np.bincount(depth, minlength=nd)
+    mf = np.bincount(depth, weights=deg[reached], minlength=nd)
+    explored = np.cumsum(mf)
+    out, prev, prevn = [], -1, 0
+    for d in range(nd):
+        if prev != BU:
+            c = BU if mf[d] > (g.edge_count - explored[d]) / 14.0 else TD
+        else:
+            c = BU if (nf[d] >= prevn or nf[d] > g.vertex_count / 24.0) else TD
+        out.append(c)
+        prev, prevn = c, nf[d]
+    return out
+
+
+def test_beamer_policy(cases):
+    """The direction-optimising baseline: exact depths and counts, and the
+    per-level choices the rule gives on the oracle's frontiers."""
+    for g, csr, dg, eng, roots in cases:
+        for root in roots:
+            print(f"case {g.name} root={root} beamer", flush=True)
+            r = eng.run(root, "beamer", "td", levels_per_sync=8)
+            want, counts = _want(csr, root)
+            assert np.array_equal(r.levels, want), (g.name, root)
+            assert np.array_equal(r.counts, counts), (g.name, root)
+            assert r.choices.tolist() == _beamer_choices(g, want), (g.name, root)
+
+
+def test_edge_list_variants_need_their_representation(gpu):
+    """Without the edge lists resident the edge-list families are refused
+    as fixed choices and never picked by the learned policy."""
+    g = graph.rmat(10, 8, seed=31)
+    csr = O.CSR.from_graph(g)
+    dg = device.DeviceGraph(g)
+    eng = bfs_native.BFSEngine(dg)
+    try:
+        assert eng.available == bfs_native.NATIVE_VARIANTS + bfs_native.REFERENCE_VARIANTS[4:]
+        with pytest.raises(N.BelewitteError):
+            eng.run(0, "fixed", "edge-list")
+        # A model that predicts the edge-list family fastest at every level.
+        m = switch.Model.init(np.random.default_rng(0).normal(8, 3, (64, 8)), seed=1, target_mean=10.0)
+        L = switch._layout(m.K)
+        m.w[L["W3"]:L["b3"]] = 0.0
+        m.w[L["b3"]:L["end"]] = 10.0
+        m.w[L["b3"] + bfs_native.variant_id("edge-list")] = 1.0
+        m.w[L["b3"] + bfs_native.variant_id("vertex-pull")] = 2.0
+        eng.set_model(m.w)
+        want, counts = _want(csr, 0)
+        r = eng.run(0, "learned", "td")
+        assert np.array_equal(r.levels, want) and np.array_equal(r.counts, counts)
+        assert np.all(r.choices == bfs_native.variant_id("vertex-pull"))
+        ev = bfs_native.BFSEngine(dg, edge_lists=True)
+        ev.set_model(m.w)
+        r = ev.run(0, "learned", "td")
+        assert np.array_equal(r.levels, want)
+        assert np.all(r.choices == bfs_native.variant_id("edge-list"))
+        ev.close()
+    finally:
+        eng.close()
+        dg.free()
+
+
+def test_native_model_still_loads(cases):
+    """A model with one output per native variant picks among those."""
+    g, csr, dg, eng, roots = cases[0]
+    m = switch.Model.init(np.random.default_rng(2).normal(8, 3, (64, 8)), outputs=3, seed=3, target_mean=10.0)
+    eng.set_model(m.w)
+    r = eng.run(roots[0], "learned", "td")
+    want, counts = _want(csr, roots[0])
+    assert np.array_equal(r.levels, want) and np.array_equal(r.counts, counts)
+    assert np.all(r.choices < 3)
+
+
 def test_exhaustive_records(cases):
     for g, csr, dg, eng, roots in cases[:5]:
         root = roots[-1]
@@ -215,7 +299,7 @@ def test_multi_root_union_matches_per_graph_bfs(gpu, variant):
               for i in range(24)]
     u, base = _union(graphs)
     dg = device.DeviceGraph(u)
-    eng = bfs_native.BFSEngine(dg)
+    eng = bfs_native.BFSEngine(dg, edge_lists=True)
     roots = [base[i] + g.max_out_degree_vertex() for i, g in enumerate(graphs)]
     if variant == "learned":
         eng.set_model(switch.Model.init(rng.normal(6, 3, (64, 8)), seed=2, target_mean=8.0).w)
