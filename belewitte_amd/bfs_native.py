@@ -30,9 +30,10 @@ VARIANTS = NATIVE_VARIANTS + REFERENCE_VARIANTS
 EDGE_LIST_REPRESENTATION = {"edge-list": "edge_list", "rev-edge-list": "rev_edge_list",
                             "struct-edge-list": "struct_edge_list", "rev-struct-edge-list": "rev_struct_edge_list"}
 POLICIES = {"fixed": 0, "schedule": 1, "learned": 2, "host": 3, "beamer": 4}
-NUM_FEATURES = 8
+NUM_FEATURES = 11
 FEATURE_NAMES = ("log2_frontier_fraction", "log2_frontier_edge_fraction", "log2_unvisited_fraction", "depth",
-                 "log2_mean_degree", "log2_std_degree", "prev_variant", "log2_prev_time_ns")
+                 "log2_mean_degree", "log2_std_degree", "prev_variant", "log2_prev_time_ns", "log2_vertices",
+                 "log2_unexplored_edge_fraction", "log2_frontier_growth")
 UNVISITED = 2147483647
 DEFAULT_HUB_DEGREE = 4096
 DEFAULT_WAVE_DEGREE = 8
@@ -71,7 +72,7 @@ class LevelRecord:
     """One level of an exhaustive run: its features and every variant's time."""
     source: int
     depth: int
-    features: np.ndarray        # float32[8]
+    features: np.ndarray        # float32[NUM_FEATURES]
     ms: np.ndarray              # float64[K] device time of each variant's level kernel
     count: int                  # vertices discovered at this level
 

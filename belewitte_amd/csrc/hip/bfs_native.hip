@@ -589,15 +589,20 @@ __global__ __launch_bounds__(kBlock) void bfs_engine_step(Graph g, Buffers b)
 
 // ------------------------------------------------------------- the policy
 
-// Packed MLP: mean[8] std[8] W1[32][8] b1[32] W2[32][32] b2[32] W3[K][32] b3[K].
+// Packed MLP: mean[F] std[F] W1[32][F] b1[32] W2[32][32] b2[32] W3[K][32] b3[K],
+// F = kFeatures.
 __host__ __device__ constexpr int mlp_len(int K)
 { return 2 * kFeatures + kHidden * kFeatures + kHidden + kHidden * kHidden + kHidden + K * kHidden + K; }
 
-// The 8 features of the pending level: frontier size, frontier out-edges and
+// The features of the pending level: frontier size, frontier out-edges and
 // unvisited vertices as log2 fractions of the graph (log2((1 + x) / V),
 // log2((1 + x) / E)), which carry over between graph scales; depth;
 // log2(1 + x) of the mean and standard deviation of the frontier
-// out-degree; previous variant; log2(1 + previous level time in ns).
+// out-degree; previous variant; log2(1 + previous level time in ns); the
+// scale log2 V (the variants' relative speed shifts with how much of the
+// bitmaps the caches hold); the undiscovered vertices' out-edges as a
+// log2 fraction of E (m_u of Beamer's rule); the frontier's growth over the
+// previous level in log2.
 __host__ __device__ void features(const BfsState &st, uint64_t V, uint64_t E, float *f)
 {
     const double n = static_cast<double>(st.qcount) + static_cast<double>(st.hubcount);
@@ -613,6 +618,10 @@ __host__ __device__ void features(const BfsState &st, uint64_t V, uint64_t E, fl
     f[5] = static_cast<float>(log2(1.0 + sqrt(var)));
     f[6] = static_cast<float>(st.prev_choice);
     f[7] = static_cast<float>(log2(1.0 + static_cast<double>(st.prev_time_ns)));
+    f[8] = static_cast<float>(lv);
+    const double mu = E > st.explored_edges ? static_cast<double>(E - st.explored_edges) : 0.0;
+    f[9] = static_cast<float>(log2(1.0 + mu) - le);
+    f[10] = static_cast<float>(log2(1.0 + n) - log2(1.0 + static_cast<double>(st.prev_frontier)));
 }
 
 // One workgroup: MLP forward on the features (one output per variant, the

@@ -9,7 +9,7 @@ benchmark-analysis/model-src/Evaluate.hs:229-230).  Here:
 * :func:`collect` times every engine variant (the native ones and the
   reference's eight kernel families) at every level from the same saved
   state (exhaustive per-level timing) and records the level's features;
-* :class:`Model` is the FP32 MLP 8 -> 32 -> 32 -> K, trained by
+* :class:`Model` is the FP32 MLP 11 -> 32 -> 32 -> K, trained by
   :meth:`Model.train` with the hand-written GPU kernels (bw_mlp_train),
   predicting the log2 level time of each variant;
 * :func:`evaluate` compares, per source, the per-level oracle (sum over
@@ -32,6 +32,7 @@ from .bfs_native import NUM_FEATURES, VARIANTS, BFSEngine, teps_edges
 from .device import DeviceArray
 
 HIDDEN = 32
+TARGET_SPAN = 3.0  # log2 units: targets are capped at 8x the level's fastest time
 
 
 def num_params(outputs: int = len(VARIANTS)) -> int:
@@ -52,7 +53,7 @@ def _layout(K: int) -> dict:
 
 @dataclass
 class Samples:
-    X: np.ndarray       # float32[n][8] features
+    X: np.ndarray       # float32[n][NUM_FEATURES] features
     ms: np.ndarray      # float64[n][K] level time of each variant
     source: np.ndarray  # int64[n]
     depth: np.ndarray   # int64[n]
@@ -67,8 +68,14 @@ class Samples:
 
     @property
     def targets(self) -> np.ndarray:
-        """log2 of the level time in ns (the regression target)."""
-        return np.log2(np.maximum(self.ms, 1e-6) * 1e6).astype(np.float32)
+        """log2 of the level time in ns (the regression target), capped at
+        TARGET_SPAN above the level's fastest variant: how much slower than
+        8x the best a variant runs never changes the choice, and fitting it
+        would take capacity from the variants that compete."""
+        y = np.log2(np.maximum(self.ms, 1e-6) * 1e6)
+        if y.size:
+            y = np.minimum(y, y.min(axis=1, keepdims=True) + TARGET_SPAN)
+        return y.astype(np.float32)
 
     def concat(self, other: "Samples") -> "Samples":
         return Samples(np.concatenate([self.X, other.X]), np.concatenate([self.ms, other.ms]),
@@ -216,17 +223,21 @@ def regret(samples: Samples, choices: np.ndarray) -> float:
     return float(got / samples.ms.min(axis=1).sum() - 1.0)
 
 
-def cross_validate(groups, candidates=(25, 50, 100, 200, 400, 800), seed: int = 0) -> dict:
-    """Leave-one-graph-out regret of the model trained for each epoch count
-    in `candidates` (mean over held-out graphs): longer training fits the
-    training graphs ever closer and carries over to other graphs worse."""
+def cross_validate(groups, candidates=(25, 50, 100, 200, 400), folds: int = 5, seed: int = 0) -> dict:
+    """Held-out-graph regret of the model trained for each epoch count in
+    `candidates`: the graphs are dealt into `folds` folds (graph i to fold
+    i mod folds), each fold is held out in turn, and the regret of each
+    held-out graph is averaged.  Longer training fits the training graphs
+    ever closer and carries over to other graphs worse."""
+    folds = max(2, min(folds, len(groups)))
     table = {}
     for ep in candidates:
         r = []
-        for i, held in enumerate(groups):
-            rest = [g for j, g in enumerate(groups) if j != i]
+        for f in range(folds):
+            held = [g for i, g in enumerate(groups) if i % folds == f]
+            rest = [g for i, g in enumerate(groups) if i % folds != f]
             m, _ = fit(balanced(rest), epochs=ep, seed=seed)
-            r.append(regret(held, m.choose(held.X)))
+            r.extend(regret(h, m.choose(h.X)) for h in held)
         table[ep] = float(np.mean(r))
     return table
 
@@ -351,10 +362,13 @@ MODEL_PATH = Path(__file__).resolve().parent / "data" / "bfs_switch_model.json"
 
 
 def training_graphs(small: bool = False):
-    """The training inputs: an Erdos-Renyi graph (2^24 vertices, average
-    degree 16, undirected), a 4096 x 4096 4-neighbour grid with 1 % of its
-    edges rewired, and the directed R-MAT scale-22 graph of the PageRank
-    workload.  small=True shrinks each for quick runs."""
+    """The training inputs, across the families and scales a search meets:
+    Erdos-Renyi (undirected and directed, average degree 8 to 64), 4-neighbour
+    grids with 1 to 10 % of their edges rewired, and R-MAT (undirected and
+    directed, edge factors 8 to 32) from 2^17 to 2^24 vertices.  The
+    variants' relative speed shifts with the scale (how much of the bitmaps
+    the caches hold), so every family appears at several.  small=True keeps
+    three small graphs for quick runs."""
     from . import graph as G
     if small:
         yield G.erdos_renyi(1 << 16, 1 << 19, seed=3, undirected=True)
@@ -364,6 +378,22 @@ def training_graphs(small: bool = False):
     yield G.erdos_renyi(1 << 24, 134217728, seed=3, undirected=True)
     yield G.grid2d(4096, 4096, rewire=0.01, seed=4)
     yield G.rmat(22, 16, seed=2)
+    yield G.rmat(23, 16, seed=25, undirected=True)
+    yield G.rmat(24, 16, seed=28)
+    yield G.rmat(21, 16, seed=21, undirected=True)
+    yield G.erdos_renyi(1 << 21, 1 << 26, seed=23, undirected=True)
+    yield G.grid2d(2048, 2048, rewire=0.05, seed=24)
+    yield G.erdos_renyi(1 << 22, 1 << 25, seed=26, undirected=False)
+    yield G.rmat(20, 8, seed=22)
+    yield G.rmat(18, 32, seed=27, undirected=True)
+    yield G.rmat(17, 16, seed=29, undirected=True)
+    yield G.rmat(18, 16, seed=30)
+    yield G.erdos_renyi(1 << 18, 1 << 21, seed=31, undirected=True)
+    yield G.erdos_renyi(1 << 19, 1 << 22, seed=32, undirected=False)
+    yield G.grid2d(1024, 1024, rewire=0.01, seed=33)
+    yield G.grid2d(512, 512, rewire=0.1, seed=34)
+    yield G.rmat(20, 16, seed=35, undirected=True)
+    yield G.erdos_renyi(1 << 20, 1 << 24, seed=36, undirected=True)
 
 
 ENGINE_SOURCES = ("csrc/hip/bfs_native.hip", "csrc/hip/mlp.hip", "../include/belewitte_hip.h")
@@ -389,15 +419,22 @@ def main(argv=None) -> int:
     from . import graph as G
     ap = argparse.ArgumentParser(description="collect exhaustive per-level BFS timings and train the switch")
     ap.add_argument("--out", default=str(MODEL_PATH))
-    ap.add_argument("--samples-out", default=None)
-    ap.add_argument("--sources", type=int, default=48)
+    ap.add_argument("--samples-out", default=None, help="save the exhaustive timings (.npz, graphs in .npz.json)")
+    ap.add_argument("--samples-in", default=None, help="train on saved timings instead of collecting them")
+    ap.add_argument("--sources", type=int, default=24)
     ap.add_argument("--epochs", type=int, default=0, help="training epochs (0: chosen by cross-validation)")
     ap.add_argument("--small", action="store_true")
     args = ap.parse_args(argv)
     device.require_gpu()
     device.set_device(0)
     groups, trained_on = [], []
-    for g in training_graphs(args.small):
+    if args.samples_in:
+        saved = Samples.load(args.samples_in)
+        trained_on = json.loads(Path(args.samples_in + ".json").read_text())
+        for gi in range(len(trained_on)):
+            m = (saved.source >> 32) == gi
+            groups.append(Samples(saved.X[m], saved.ms[m], saved.source[m], saved.depth[m]))
+    for g in ([] if args.samples_in else training_graphs(args.small)):
         t0 = time.perf_counter()
         print(f"[switch] {g.name} generated", file=sys.stderr, flush=True)
         dg = device.DeviceGraph(g)
@@ -419,12 +456,13 @@ def main(argv=None) -> int:
         samples = samples.concat(s)
     if args.samples_out:
         samples.save(args.samples_out)
+        Path(args.samples_out + ".json").write_text(json.dumps(trained_on))
     epochs = args.epochs
     cv = None
     if epochs <= 0:
         cv = cross_validate(groups)
         epochs = min(cv, key=cv.get)
-        print(f"[switch] leave-one-graph-out regret by epochs: "
+        print(f"[switch] held-out-graph regret by epochs: "
               f"{ {k: round(v, 4) for k, v in cv.items()} }; training {epochs} epochs", file=sys.stderr, flush=True)
     model, loss = fit(balanced(groups), epochs=epochs)
     choice = model.choose(samples.X)
@@ -439,7 +477,7 @@ def main(argv=None) -> int:
     props = device.device_props(0)
     model.provenance = {
         "engine": engine_fingerprint(),
-        "device": f"{props['name']} ({props['arch']}, {props['compute_units']} CUs)",
+        "device": f"{props['arch']}, {props['compute_units']} CUs" + (f" ({props['name']})" if props["name"] else ""),
         "trained": datetime.datetime.now(datetime.timezone.utc).isoformat(timespec="seconds"),
         "graphs": trained_on,
         "epochs": int(epochs), "seed": 0,
@@ -447,7 +485,8 @@ def main(argv=None) -> int:
         "training_set_ms": {"oracle": float(oracle), "learned": float(learned),
                             "fixed": dict(zip(VARIANTS, np.asarray(fixed, dtype=float).tolist()))},
         "command": "python -m belewitte_amd.switch" + (" --small" if args.small else "")
-                   + f" --sources {args.sources}" + (f" --epochs {args.epochs}" if args.epochs else ""),
+                   + f" --sources {args.sources}" + (f" --epochs {args.epochs}" if args.epochs else "")
+                   + (" (trained on saved timings)" if args.samples_in else ""),
     }
     Path(args.out).parent.mkdir(parents=True, exist_ok=True)
     model.save(args.out)

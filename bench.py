@@ -194,9 +194,11 @@ def bfs_cpu_leg(g, sources, teps_div: int):
 def bfs_learned_switch(scale: int, sources: int, samples_out=None, cpu: bool = True) -> dict:
     """Undirected R-MAT (scale, edge factor 16, seed 5), `sources` random
     sources of out-degree >= 1: the learned per-level switch against the
-    best fixed native variant and the per-level oracle, simulated on
-    exhaustive per-level timings and executed end to end on the device;
-    every depth checked against the CPU baseline's (the oracle)."""
+    best fixed variant of all eleven (native and the reference's kernel
+    families), the per-level oracle over the same set and Beamer's
+    direction-optimising rule, simulated on exhaustive per-level timings and
+    executed end to end on the device; every depth checked against the CPU
+    baseline's (the oracle)."""
     from belewitte_amd import bfs_native, device, graph, switch
     t0 = time.perf_counter()
     g = graph.rmat(scale, 16, seed=5, undirected=True)
@@ -205,7 +207,7 @@ def bfs_learned_switch(scale: int, sources: int, samples_out=None, cpu: bool = T
     base, ref = bfs_cpu_leg(g, src, 2) if cpu else (None, None)
     dg = device.DeviceGraph(g)
     t0 = time.perf_counter()
-    eng = bfs_native.BFSEngine(dg)
+    eng = bfs_native.BFSEngine(dg, edge_lists=True)
     device.synchronize()
     setup_ms = (time.perf_counter() - t0) * 1e3
     if switch.MODEL_PATH.exists():
@@ -233,8 +235,9 @@ def bfs_learned_switch(scale: int, sources: int, samples_out=None, cpu: bool = T
     res["graph"] = {"name": g.name, "vertices": g.vertex_count, "stored_edges": g.edge_count,
                     "teps_convention": "undirected: reached out-degree sum / 2 (Graph500)"}
     res["engine_setup_ms"] = round(setup_ms, 2)
-    res["engine_setup_note"] = ("once per graph, outside the GTEPS figures: engine buffers and the likely-parent "
-                                "array of the bottom-up probes")
+    res["engine_setup_note"] = ("once per graph, outside the GTEPS figures: the four edge lists the reference's "
+                                "edge-list families read (built on the device from the CSR), engine buffers and "
+                                "the likely-parent array of the bottom-up probes")
     res["model"] = origin
     res["depth_check"] = "oracle (CPU baseline's depths)" if ref is not None else "none (--no-cpu)"
     if base is not None:
@@ -249,10 +252,10 @@ def bfs_training_graphs(sources: int = 16, cpu: bool = True) -> dict:
     set: an undirected Erdos-Renyi graph (2^24 vertices, average degree 16,
     seed 3) and a 4096 x 4096 4-neighbour grid with 1% of its edges rewired
     (seed 4).  For each, executed runs from `sources` random sources (seed
-    10) of the learned switch and of every fixed native variant (device
-    time summed over the sources, GTEPS in the undirected convention); the
-    learned runs' depths are checked against the oracle's from every
-    source (its parallel queue BFS, timed beside them)."""
+    10) of the learned switch, Beamer's rule and every fixed variant of the
+    eleven (device time summed over the sources, GTEPS in the undirected
+    convention); the learned runs' depths are checked against the oracle's
+    from every source (its parallel queue BFS, timed beside them)."""
     from belewitte_amd import bfs_native, device, graph, switch
     model = switch.Model.load(switch.MODEL_PATH)
     threads, _, _ = host_cpus()
@@ -266,7 +269,7 @@ def bfs_training_graphs(sources: int = 16, cpu: bool = True) -> dict:
             from oracle import binding as O
             ref, ref_s = oracle_depths(O.CSR.from_graph(g), src, threads)
         dg = device.DeviceGraph(g)
-        eng = bfs_native.BFSEngine(dg)
+        eng = bfs_native.BFSEngine(dg, edge_lists=True)
         eng.set_model(model.w)
         edges, levels, mism = 0, 0, 0
         for i, s in enumerate(src):
@@ -276,18 +279,19 @@ def bfs_training_graphs(sources: int = 16, cpu: bool = True) -> dict:
             if ref is not None and not np.array_equal(r.levels, ref[i]):
                 mism += 1
         ms = {}
-        for pol, var in [("learned", "td")] + [("fixed", v) for v in bfs_native.VARIANTS]:
+        for pol, var in [("learned", "td"), ("beamer", "td")] + [("fixed", v) for v in bfs_native.VARIANTS]:
             eng.run(src[0], pol, var, download=False)
-            ms["learned" if pol == "learned" else var] = sum(
-                eng.run(s, pol, var, download=False).total_ms for s in src)
-        fixed = {k: v for k, v in ms.items() if k != "learned"}
+            ms[pol if pol != "fixed" else var] = sum(eng.run(s, pol, var, download=False).total_ms for s in src)
+        fixed = {k: v for k, v in ms.items() if k not in ("learned", "beamer")}
         best = min(fixed, key=fixed.get)
         out[g.name] = {"vertices": g.vertex_count, "stored_edges": g.edge_count, "sources": len(src),
                        "levels": levels, "teps_edges": edges, "executed_ms": {k: round(v, 3) for k, v in ms.items()},
                        "gteps": {"learned": round(edges / (ms["learned"] / 1e3) / 1e9, 2),
+                                 "beamer": round(edges / (ms["beamer"] / 1e3) / 1e9, 2),
                                  "best_fixed": round(edges / (fixed[best] / 1e3) / 1e9, 2)},
                        "best_fixed_variant": best,
                        "speedup_vs_best_fixed": round(fixed[best] / ms["learned"], 3),
+                       "speedup_vs_beamer": round(ms["beamer"] / ms["learned"], 3),
                        "depth_mismatches_vs_oracle": mism if ref is not None else None,
                        "oracle_queue_bfs_s": round(ref_s, 3) if ref_s is not None else None}
         log(f"bfs {g.name}: {out[g.name]['executed_ms']} ({time.perf_counter() - t0:.1f}s)")

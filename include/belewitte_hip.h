@@ -552,13 +552,15 @@ typedef struct bw_bfs_engine bw_bfs_engine;
  * fewer than V / beta vertices; else bottom-up. */
 #define BW_POLICY_BEAMER 4
 
-/* Features of a level, reduced on the device: frontier vertices, frontier
- * out-edges and unvisited vertices as log2 fractions of the graph
- * (log2((1 + x) / V), log2((1 + x) / E), log2((1 + x) / V)); depth;
- * log2(1 + x) of the mean and standard deviation of the frontier
- * out-degree; previous variant (-1 at level 0); log2(1 + previous level
- * time in ns). */
-#define BW_BFS_FEATURES 8
+/* Features of a level, reduced on the device: [0..2] frontier vertices,
+ * frontier out-edges and unvisited vertices as log2 fractions of the graph
+ * (log2((1 + x) / V), log2((1 + x) / E), log2((1 + x) / V)); [3] depth;
+ * [4..5] log2(1 + x) of the mean and standard deviation of the frontier
+ * out-degree; [6] previous variant (-1 at level 0); [7] log2(1 + previous
+ * level time in ns); [8] log2 V; [9] out-edges of the vertices not yet
+ * discovered, log2((1 + m_u) / E); [10] frontier growth, log2(1 + n_f) -
+ * log2(1 + previous frontier size) (that size is 0 at level 0). */
+#define BW_BFS_FEATURES 11
 
 /* levels: device int32[V] receiving the depths, or NULL for an
  * engine-owned array (bw_bfs_engine_levels).  max_levels bounds the depth
@@ -580,8 +582,8 @@ int bw_bfs_engine_set_tuning(bw_bfs_engine *engine, uint32_t hub_degree,
 /* Most roots of one multi-root search. */
 #define BW_BFS_MAX_ROOTS 16384
 
-/* Float count of a packed model with `outputs` outputs: mean[8] std[8]
- * W1[32][8] b1[32] W2[32][32] b2[32] W3[K][32] b3[K] (row-major, ReLU
+/* Float count of a packed model with `outputs` outputs: mean[11] std[11]
+ * W1[32][11] b1[32] W2[32][32] b2[32] W3[K][32] b3[K] (row-major, ReLU
  * hidden layers, outputs = predicted log2 level time per variant; the
  * argmin is taken over the variants the engine can run, i.e. those whose
  * representation its graph holds).  outputs is BW_BFS_NATIVE_VARIANTS or

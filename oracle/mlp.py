@@ -1,4 +1,4 @@
-"""FP64 restatement of the BFS variant predictor (8 -> 32 -> 32 -> K MLP).
+"""FP64 restatement of the BFS variant predictor (11 -> 32 -> 32 -> K MLP).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/ as the checker of the GPU
 inference, gradient and training kernels -- never by the product path.
@@ -14,7 +14,7 @@ from __future__ import annotations
 
 import numpy as np
 
-F, H = 8, 32
+F, H = 11, 32  # features (BW_BFS_FEATURES), hidden width
 
 
 def layout(K: int) -> dict:

@@ -111,7 +111,7 @@ def test_schedules_bit_exact(cases, schedule):
 
 def test_random_model_never_changes_depths(cases):
     for seed in range(3):
-        m = switch.Model.init(np.random.default_rng(seed).normal(8, 3, (64, 8)), seed=seed, target_mean=10.0)
+        m = switch.Model.init(np.random.default_rng(seed).normal(8, 3, (64, bfs_native.NUM_FEATURES)), seed=seed, target_mean=10.0)
         for g, csr, dg, eng, roots in cases:
             eng.set_model(m.w)
             for root in roots[:2]:
@@ -157,11 +157,15 @@ def test_device_features_match_host(cases):
             mean = fr.mean() if n else 0.0
             std = np.sqrt(max((fr ** 2).mean() - mean * mean, 0.0)) if n else 0.0
             lv, le = np.log2(V), np.log2(max(g.edge_count, 1))
+            explored = deg[(levels <= d) & (levels != bfs_native.UNVISITED)].sum()
+            prevn = np.count_nonzero(levels == d - 1) if d else 0
             want = [np.log2(1 + n) - lv, np.log2(1 + fr.sum()) - le,
                     np.log2(1 + V - np.count_nonzero(levels <= d)) - lv, d, np.log2(1 + mean), np.log2(1 + std)]
             assert np.allclose(f[:6], want, rtol=1e-5, atol=1e-5), (g.name, d, f, want)
             assert f[6] == (-1 if d == 0 else (d - 1) % 3)
             assert (f[7] == 0) == (d == 0)
+            want = [lv, np.log2(1 + g.edge_count - explored) - le, np.log2(1 + n) - np.log2(1 + prevn)]
+            assert np.allclose(f[8:], want, rtol=1e-5, atol=1e-5), (g.name, d, f, want)
             N.check_hip(lib.bw_bfs_engine_force(eng._e, d % 3, None))
             N.check_hip(lib.bw_bfs_engine_level(eng._e, bfs_native.POLICIES["host"], 0, 1, None))
             d += 1
@@ -171,7 +175,7 @@ def test_device_choice_matches_host_model(cases):
     """The MLP evaluated inside the level loop picks what the inference
     kernel picks on the same features."""
     lib = N.hip_lib()
-    X = np.random.default_rng(1).normal(10, 4, (256, 8)).astype(np.float32)
+    X = np.random.default_rng(1).normal(10, 4, (256, bfs_native.NUM_FEATURES)).astype(np.float32)
     m = switch.Model.init(X, seed=5, target_mean=12.0)
     for g, csr, dg, eng, roots in cases[:4]:
         eng.set_model(m.w)
@@ -234,7 +238,7 @@ def test_edge_list_variants_need_their_representation(gpu):
         with pytest.raises(N.BelewitteError):
             eng.run(0, "fixed", "edge-list")
         # A model that predicts the edge-list family fastest at every level.
-        m = switch.Model.init(np.random.default_rng(0).normal(8, 3, (64, 8)), seed=1, target_mean=10.0)
+        m = switch.Model.init(np.random.default_rng(0).normal(8, 3, (64, bfs_native.NUM_FEATURES)), seed=1, target_mean=10.0)
         L = switch._layout(m.K)
         m.w[L["W3"]:L["b3"]] = 0.0
         m.w[L["b3"]:L["end"]] = 10.0
@@ -259,7 +263,7 @@ def test_edge_list_variants_need_their_representation(gpu):
 def test_native_model_still_loads(cases):
     """A model with one output per native variant picks among those."""
     g, csr, dg, eng, roots = cases[0]
-    m = switch.Model.init(np.random.default_rng(2).normal(8, 3, (64, 8)), outputs=3, seed=3, target_mean=10.0)
+    m = switch.Model.init(np.random.default_rng(2).normal(8, 3, (64, bfs_native.NUM_FEATURES)), outputs=3, seed=3, target_mean=10.0)
     eng.set_model(m.w)
     r = eng.run(roots[0], "learned", "td")
     want, counts = _want(csr, roots[0])
@@ -302,7 +306,7 @@ def test_multi_root_union_matches_per_graph_bfs(gpu, variant):
     eng = bfs_native.BFSEngine(dg, edge_lists=True)
     roots = [base[i] + g.max_out_degree_vertex() for i, g in enumerate(graphs)]
     if variant == "learned":
-        eng.set_model(switch.Model.init(rng.normal(6, 3, (64, 8)), seed=2, target_mean=8.0).w)
+        eng.set_model(switch.Model.init(rng.normal(6, 3, (64, bfs_native.NUM_FEATURES)), seed=2, target_mean=8.0).w)
         r = eng.run(roots, "learned", "td")
     else:
         r = eng.run(roots, "fixed", variant)

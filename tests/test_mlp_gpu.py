@@ -16,7 +16,8 @@ K = 3
 
 def _data(n, seed):
     rng = np.random.default_rng(seed)
-    X = rng.normal(0, 1, (n, 8)) * np.array([3, 4, 2, 5, 1, 1, 1, 2]) + np.array([10, 14, 20, 6, 3, 2, 1, 12])
+    X = rng.normal(0, 1, (n, 11)) * np.array([3, 4, 2, 5, 1, 1, 1, 2, 1, 2, 1]) \
+        + np.array([10, 14, 20, 6, 3, 2, 1, 12, 20, -3, 0])
     return X.astype(np.float32)
 
 
@@ -37,7 +38,7 @@ def test_gradient_matches_finite_differences(gpu):
     g, loss = m.gradient(X, Y)
     assert abs(loss - OM.loss(m.w, K, X, Y)) <= 1e-4 * OM.loss(m.w, K, X, Y)
     fd = OM.fd_gradient(m.w, K, X, Y, h=1e-4)
-    assert np.all(g[:16] == 0)
+    assert np.all(g[:2 * OM.F] == 0)
     err = np.linalg.norm(g.astype(np.float64) - fd)
     assert err <= 2e-3 * np.linalg.norm(fd), (err, np.linalg.norm(fd))
     # Each block of parameters on its own (a wrong index inside one layer
@@ -73,4 +74,4 @@ def test_training_is_reproducible(gpu):
     a.train(X, Y, epochs=50, batch=32, lr=1e-3, seed=2)
     b.train(X, Y, epochs=50, batch=32, lr=1e-3, seed=2)
     assert np.array_equal(a.w.view(np.uint32), b.w.view(np.uint32))
-    assert np.array_equal(a.w[:16], switch.Model.init(X, K, seed=1).w[:16])  # mean/std untouched
+    assert np.array_equal(a.w[:2 * OM.F], switch.Model.init(X, K, seed=1).w[:2 * OM.F])  # mean/std untouched

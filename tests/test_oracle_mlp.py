@@ -14,8 +14,8 @@ def _identity_net(shift=(1.0, 2.0, 3.0), mean=0.0, std=1.0):
     w = np.zeros(L["end"])
     w[L["mean"]:L["std"]] = mean
     w[L["std"]:L["W1"]] = std
-    W1 = np.zeros((32, 8))
-    W1[:8, :8] = np.eye(8)
+    W1 = np.zeros((32, OM.F))
+    W1[:OM.F, :OM.F] = np.eye(OM.F)
     w[L["W1"]:L["b1"]] = W1.ravel()
     w[L["W2"]:L["b2"]] = np.eye(32).ravel()
     W3 = np.zeros((K, 32))
@@ -26,12 +26,13 @@ def _identity_net(shift=(1.0, 2.0, 3.0), mean=0.0, std=1.0):
 
 
 def test_layout_sizes():
-    assert OM.layout(3)["end"] == 16 + 256 + 32 + 1024 + 32 + 96 + 3
-    assert OM.layout(16)["end"] == 16 + 256 + 32 + 1024 + 32 + 512 + 16
+    assert OM.F == 11
+    assert OM.layout(3)["end"] == 22 + 352 + 32 + 1024 + 32 + 96 + 3
+    assert OM.layout(16)["end"] == 22 + 352 + 32 + 1024 + 32 + 512 + 16
 
 
 def test_forward_closed_form():
-    X = np.array([[-1.0, 2.0, 0.5, 7, 7, 7, 7, 7], [3.0, -4.0, -0.25, 0, 0, 0, 0, 0]])
+    X = np.array([[-1.0, 2.0, 0.5] + [7.0] * (OM.F - 3), [3.0, -4.0, -0.25] + [0.0] * (OM.F - 3)])
     out = OM.forward(_identity_net(), K, X)
     want = np.maximum(X[:, :K], 0) + np.array([1.0, 2.0, 3.0])
     assert np.allclose(out, want, atol=0, rtol=0)
@@ -42,7 +43,7 @@ def test_forward_closed_form():
 
 def test_fd_gradient_of_output_bias():
     rng = np.random.default_rng(0)
-    X = rng.normal(0, 2, (20, 8))
+    X = rng.normal(0, 2, (20, OM.F))
     Y = rng.normal(0, 1, (20, K))
     w = _identity_net()
     L = OM.layout(K)

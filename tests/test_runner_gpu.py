@@ -151,12 +151,12 @@ def test_switch_with_predictor_and_log(gfile, tmp_path):
 def test_native_engine_through_the_runner(gfile, tmp_path):
     """The device-driven engine as kernel-runner implementations: fixed
     variants, a schedule and the learned switch with a model file."""
-    from belewitte_amd import switch
+    from belewitte_amd import bfs_native, switch
     g, p = gfile
     root = g.max_out_degree_vertex()
     want, nlev = O.bfs_queue(O.CSR.from_graph(g), root)
     model = tmp_path / "m.json"
-    switch.Model.init(np.random.default_rng(0).normal(8, 3, (32, 8)), seed=1, target_mean=9.0).save(model)
+    switch.Model.init(np.random.default_rng(0).normal(8, 3, (32, bfs_native.NUM_FEATURES)), seed=1, target_mean=9.0).save(model)
     jobs = [("native", ["--variant", "td"]), ("native", ["--variant", "bu-wave"]),
             ("native-schedule", ["--schedule", "bu,td,bu-wave"]), ("native-learned", ["--model", str(model)])]
     for i, (name, extra) in enumerate(jobs):
