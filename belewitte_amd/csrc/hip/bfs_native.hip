@@ -624,64 +624,145 @@ __host__ __device__ void features(const BfsState &st, uint64_t V, uint64_t E, fl
     f[10] = static_cast<float>(log2(1.0 + n) - log2(1.0 + static_cast<double>(st.prev_frontier)));
 }
 
-// One workgroup: MLP forward on the features (one output per variant, the
-// predicted log2 time), returns the argmin over the variants this graph can
-// run (-1 if none).  `w` is the workgroup's LDS copy of the packed weights
-// (see stage_model).
-__device__ int32_t mlp_choose(const Graph &g, const float *w, int K, const float *f, float *h1, float *h2,
-                              float *out)
+// The model in wave 0 of a kernel, weights in registers: lane j holds row
+// j % kHidden of W1 and W2 and lane k row min(k, K - 1) of W3 (lanes past
+// the rows hold copies).  Loaded at the kernel's start so that the loads'
+// latency overlaps the kernel's other work.
+struct LaneModel {
+    float w1[kFeatures], b1;
+    float w2[kHidden], b2;
+    float w3[kHidden], b3;
+};
+
+__device__ __forceinline__ void load_lane_model(const PolicyArgs &p, LaneModel &m)
 {
-    const float *mean = w, *sd = w + kFeatures;
-    const float *W1 = sd + kFeatures, *b1 = W1 + kHidden * kFeatures;
+    if (p.policy != LEARNED) return;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int j = lane & (kHidden - 1), k = min(lane, p.nout - 1);
+    const float *W1 = p.mlp + 2 * kFeatures, *b1 = W1 + kHidden * kFeatures;
     const float *W2 = b1 + kHidden, *b2 = W2 + kHidden * kHidden;
-    const float *W3 = b2 + kHidden, *b3 = W3 + K * kHidden;
-    const int t = threadIdx.x;
-    if (t < kHidden) {
-        float a = b1[t];
-        for (int i = 0; i < kFeatures; i++) a += W1[t * kFeatures + i] * ((f[i] - mean[i]) / sd[i]);
-        h1[t] = fmaxf(a, 0.0f);
-    }
-    __syncthreads();
-    if (t < kHidden) {
-        float a = b2[t];
-        for (int i = 0; i < kHidden; i++) a += W2[t * kHidden + i] * h1[i];
-        h2[t] = fmaxf(a, 0.0f);
-    }
-    __syncthreads();
-    if (t < K) {
-        float a = b3[t];
-        for (int i = 0; i < kHidden; i++) a += W3[t * kHidden + i] * h2[i];
-        out[t] = a;
-    }
-    __syncthreads();
+    const float *W3 = b2 + kHidden, *b3 = W3 + p.nout * kHidden;
+#pragma unroll
+    for (int i = 0; i < kFeatures; i++) m.w1[i] = W1[j * kFeatures + i];
+    m.b1 = b1[j];
+#pragma unroll
+    for (int i = 0; i < kHidden; i++) m.w2[i] = W2[j * kHidden + i];
+    m.b2 = b2[j];
+#pragma unroll
+    for (int i = 0; i < kHidden; i++) m.w3[i] = W3[k * kHidden + i];
+    m.b3 = b3[k];
+}
+
+// Lane i's value of x, read by the whole wave (v_readlane; i is uniform and
+// every lane is active).
+__device__ __forceinline__ float lane_value(float x, int i)
+{
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), i));
+}
+
+__device__ __forceinline__ double lane_value(double x, int i)
+{
+    const uint64_t u = static_cast<uint64_t>(__double_as_longlong(x));
+    const uint32_t lo = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(u & 0xFFFFFFFFu), i));
+    const uint32_t hi = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(u >> 32), i));
+    return __longlong_as_double(static_cast<long long>((static_cast<uint64_t>(hi) << 32) | lo));
+}
+
+// Wave 0: MLP forward on the features fv (the same in every lane; one
+// output per variant, the predicted log2 time) and the argmin -- the first
+// minimum -- over the variants this graph can run (-1 if none).  The sums
+// run in the order of the batch inference kernel, so both pick alike.
+__device__ int32_t mlp_choose_wave(const Graph &g, const PolicyArgs &p, const LaneModel &m, const float *fv)
+{
+    const float *mean = p.mlp, *sd = p.mlp + kFeatures;
+    float a = m.b1;
+#pragma unroll
+    for (int i = 0; i < kFeatures; i++) a += m.w1[i] * ((fv[i] - mean[i]) / sd[i]);
+    const float h1 = fmaxf(a, 0.0f);
+    a = m.b2;
+#pragma unroll
+    for (int i = 0; i < kHidden; i++) a += m.w2[i] * lane_value(h1, i);
+    const float h2 = fmaxf(a, 0.0f);
+    a = m.b3;
+#pragma unroll
+    for (int i = 0; i < kHidden; i++) a += m.w3[i] * lane_value(h2, i);
     int32_t best = -1;
-    for (int k = 0; k < K; k++)
-        if (variant_available(g, k) && (best < 0 || out[k] < out[best])) best = k;
+    float bv = 0.0f;
+    for (int k = 0; k < p.nout; k++) {
+        const float v = lane_value(a, k);
+        if (variant_available(g, k) && (best < 0 || v < bv)) {
+            best = k;
+            bv = v;
+        }
+    }
     return best;
 }
 
-// Copies the packed weights into LDS; issued first in a kernel so the loads
-// overlap the work before the first barrier.
-__device__ __forceinline__ void stage_model(const PolicyArgs &p, float *sw)
+// features() spread over wave 0: lane 0 reads the state and forms the ten
+// logarithms' arguments, lane i takes the i-th logarithm, and every lane
+// assembles the same features from them (one double-precision log2 of
+// latency instead of ten in a row).
+__device__ void features_wave(const BfsState &st, uint64_t V, uint64_t E, float *f)
 {
-    if (p.policy != LEARNED) return;
-    const int n = mlp_len(p.nout);
-    for (int i = threadIdx.x; i < n; i += kBlock) sw[i] = p.mlp[i];
+    const int lane = threadIdx.x & (kWave - 1);
+    double arg[10] = {1, 1, 1, 1, 1, 1, 1, 1, 1, 1};
+    double depth = 0.0, prev = 0.0;
+    if (lane == 0) {
+        const double n = static_cast<double>(st.qcount) + static_cast<double>(st.hubcount);
+        const double edges = static_cast<double>(st.frontier_edges);
+        const double mean = n > 0 ? edges / n : 0.0;
+        const double var = n > 0 ? fmax(st.frontier_sq / n - mean * mean, 0.0) : 0.0;
+        arg[0] = static_cast<double>(V);
+        arg[1] = static_cast<double>(E > 0 ? E : 1);
+        arg[2] = 1.0 + n;
+        arg[3] = 1.0 + edges;
+        arg[4] = 1.0 + static_cast<double>(V - st.visited);
+        arg[5] = 1.0 + mean;
+        arg[6] = 1.0 + sqrt(var);
+        arg[7] = 1.0 + static_cast<double>(st.prev_time_ns);
+        arg[8] = 1.0 + (E > st.explored_edges ? static_cast<double>(E - st.explored_edges) : 0.0);
+        arg[9] = 1.0 + static_cast<double>(st.prev_frontier);
+        depth = static_cast<double>(st.depth);
+        prev = static_cast<double>(st.prev_choice);
+    }
+    double mine = 1.0;
+#pragma unroll
+    for (int i = 0; i < 10; i++) {
+        const double a = lane_value(arg[i], 0);
+        if (lane == i) mine = a;
+    }
+    const double l = log2(mine);
+    double lg[10];
+#pragma unroll
+    for (int i = 0; i < 10; i++) lg[i] = lane_value(l, i);
+    const double lv = lg[0], le = lg[1];
+    f[0] = static_cast<float>(lg[2] - lv);
+    f[1] = static_cast<float>(lg[3] - le);
+    f[2] = static_cast<float>(lg[4] - lv);
+    f[3] = static_cast<float>(lane_value(depth, 0));
+    f[4] = static_cast<float>(lg[5]);
+    f[5] = static_cast<float>(lg[6]);
+    f[6] = static_cast<float>(lane_value(prev, 0));
+    f[7] = static_cast<float>(lg[7]);
+    f[8] = static_cast<float>(lv);
+    f[9] = static_cast<float>(lg[8] - le);
+    f[10] = static_cast<float>(lg[2] - lg[9]);
 }
 
-// Block-uniform; the result is valid in every thread.  sw holds the staged
-// weights (stage_model, followed by at least one barrier).
-__device__ int32_t choose(const PolicyArgs &p, const BfsState &st, const Graph &g, const float *sw, float *f,
-                          float *h1, float *h2, float *out)
+// Wave 0, after lane 0 has brought the state to the pending level; the
+// result is valid in lane 0.
+__device__ int32_t choose(const PolicyArgs &p, const BfsState &st, const Graph &g, const LaneModel &m)
 {
-    switch (p.policy) {
-    case SCHEDULE: return p.schedule[st.depth % static_cast<uint32_t>(p.schedule_len)];
-    case LEARNED: {
-        if (threadIdx.x == 0) features(st, g.V, g.E, f);
-        __syncthreads();
-        const int32_t c = mlp_choose(g, sw, p.nout, f, h1, h2, out);
+    const int lane = threadIdx.x & (kWave - 1);
+    if (p.policy == LEARNED) {
+        float fv[kFeatures];
+        features_wave(st, g.V, g.E, fv);
+        const int32_t c = mlp_choose_wave(g, p, m, fv);
         return c >= 0 ? c : p.fixed;
     }
+    if (lane != 0) return p.fixed;
+    switch (p.policy) {
+    case SCHEDULE: return p.schedule[st.depth % static_cast<uint32_t>(p.schedule_len)];
     case HOST: return st.choice;
     case BEAMER: {
         // After a top-down level: bottom-up once the frontier's out-edges
@@ -700,6 +781,17 @@ __device__ int32_t choose(const PolicyArgs &p, const BfsState &st, const Graph &
     }
 }
 
+// Wave 0: picks the pending level's variant unless the search is done.
+__device__ __forceinline__ void choose_next(const PolicyArgs &p, BfsState &st, const Graph &g, const LaneModel &m)
+{
+    const int lane = threadIdx.x & (kWave - 1);
+    uint32_t done = 0;
+    if (lane == 0) done = st.done;
+    if (__shfl(done, 0, kWave)) return;
+    const int32_t c = choose(p, st, g, m);
+    if (lane == 0) st.choice = c;
+}
+
 // The per-level epilogue: one workgroup.  Records the finished level,
 // advances the state to the next frontier and picks its variant.
 __global__ __launch_bounds__(kBlock) void bfs_engine_epilogue(Graph g, Buffers b, PolicyArgs p, uint32_t nparts)
@@ -707,14 +799,13 @@ __global__ __launch_bounds__(kBlock) void bfs_engine_epilogue(Graph g, Buffers b
     __shared__ uint64_t rout[kBlock / kWave];
     __shared__ double rsq[kBlock / kWave];
     __shared__ uint64_t hscan[kBlock];
-    __shared__ float f[kFeatures], h1[kHidden], h2[kHidden], out[kMaxOut];
-    __shared__ float sw[mlp_len(kMaxOut)];
 
     BfsState &st = *b.st;
     if (st.done) return;
     const uint64_t now = wall_clock64();
-    stage_model(p, sw);
     const int tid = threadIdx.x, lane = tid & (kWave - 1), wid = tid / kWave;
+    LaneModel lm;
+    if (wid == 0) load_lane_model(p, lm);
     uint64_t os = 0;
     double sq = 0.0;
     for (uint32_t i = tid; i < nparts; i += kBlock) {
@@ -800,12 +891,7 @@ __global__ __launch_bounds__(kBlock) void bfs_engine_epilogue(Graph g, Buffers b
             st.fill_levels = deferred ? 1u : 0u;
         }
     }
-    __syncthreads();
-    if (!st.done) {
-        const int32_t c = choose(p, st, g, sw, f, h1, h2, out);
-        __syncthreads();
-        if (tid == 0) st.choice = c;
-    }
+    if (wid == 0) choose_next(p, st, g, lm);
 }
 
 // Level 0: the roots (roots[0..nroots), or just `root` when roots is NULL)
@@ -813,10 +899,9 @@ __global__ __launch_bounds__(kBlock) void bfs_engine_epilogue(Graph g, Buffers b
 __global__ __launch_bounds__(kBlock) void bfs_engine_init(Graph g, Buffers b, PolicyArgs p, uint32_t root,
                                                           const uint32_t *roots, uint32_t nroots)
 {
-    __shared__ float f[kFeatures], h1[kHidden], h2[kHidden], out[kMaxOut];
-    __shared__ float sw[mlp_len(kMaxOut)];
     BfsState &st = *b.st;
-    stage_model(p, sw);
+    LaneModel lm;
+    if (threadIdx.x < kWave) load_lane_model(p, lm);
     if (threadIdx.x == 0) {
         uint32_t nq = 0, nh = 0;
         uint64_t visited = 0, edges = 0;
@@ -858,10 +943,7 @@ __global__ __launch_bounds__(kBlock) void bfs_engine_init(Graph g, Buffers b, Po
         st.hubcount = nh;
         st.choice = p.fixed;
     }
-    __syncthreads();
-    const int32_t c = choose(p, st, g, sw, f, h1, h2, out);
-    __syncthreads();
-    if (threadIdx.x == 0) st.choice = c;
+    if (threadIdx.x < kWave) choose_next(p, st, g, lm);
 }
 
 __global__ void set_choice(BfsState *st, int32_t c) { st->choice = c; }
