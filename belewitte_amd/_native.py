@@ -187,6 +187,7 @@ _HIP_SIGS = {
     "bw_bfs_engine_run": (C.c_int, [vp, C.c_uint32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
                                     u32p, u64p, i32p, u32p, f32p, vp]),
     "bw_bfs_engine_set_tuning": (C.c_int, [vp, C.c_uint32, C.c_uint32]),
+    "bw_bfs_engine_set_queues": (C.c_int, [vp, C.c_uint32, C.c_uint32]),
     "bw_bfs_engine_begin_multi": (C.c_int, [vp, u32p, C.c_uint32, C.c_int32, C.c_int32, C.c_int32, vp]),
     "bw_bfs_engine_run_multi": (C.c_int, [vp, u32p, C.c_uint32, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
                                           C.c_int32, u32p, u64p, i32p, u32p, f32p, vp]),

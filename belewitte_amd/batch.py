@@ -10,9 +10,11 @@ captured hipGraph batches (no host round trip per level):
   each component gets exactly its own BFS depths.
 
 The autotuner times whole batch runs over a grid of the kernels'
-thread/wave/workgroup thresholds and keeps the fastest:
+thread/wave/workgroup thresholds and queue sizes and keeps the fastest:
 - the engine: hub out-degree for workgroup-sliced top-down expansion and
-  in-degree for wave-wide bottom-up probes;
+  in-degree for wave-wide bottom-up probes, then the discoveries staged
+  per workgroup in LDS (the level kernel's dynamic LDS, against its
+  occupancy) and the hub-queue capacity;
 - the multi-source search: the divisor that picks top-down levels, the
   in-degree above which a vertex's in-edges are split into wave-sized
   chunks, and the in-degree for wave-wide probes.
@@ -30,6 +32,8 @@ from .bfs_native import UNVISITED, BFSEngine, MultiSourceBFS
 
 HUB_CANDIDATES = (1024, 4096, 16384)
 WAVE_CANDIDATES = (8, 16, 32, 64, 128)
+LDS_QUEUE_CANDIDATES = (512, 1024, 2048, 4096, 8192)
+HUB_CAP_CANDIDATES = (1024, 4096, 16384)
 TOPDOWN_DIVISORS = (0, 1, 2, 4, 8, 16)
 MSBFS_HUB_CANDIDATES = (32, 64, 128, 256, 1024)
 
@@ -74,21 +78,33 @@ class Tuned:
 
 
 def autotune_engine(eng: BFSEngine, roots, policy: str = "fixed", variant="td", runs: int = 2,
-                    hubs=HUB_CANDIDATES, waves=WAVE_CANDIDATES) -> Tuned:
-    """Grid search of (hub_degree, wave_degree) on whole runs from `roots`
-    (a root, or a list for one multi-root search); leaves the engine set
-    to the fastest."""
+                    hubs=HUB_CANDIDATES, waves=WAVE_CANDIDATES, lds_queues=LDS_QUEUE_CANDIDATES,
+                    hub_caps=HUB_CAP_CANDIDATES) -> Tuned:
+    """On whole runs from `roots` (a root, or a list for one multi-root
+    search): a grid search of (hub_degree, wave_degree) at the default
+    queue sizes, then the LDS queue size, then the hub-queue capacity;
+    leaves the engine set to the fastest (hub_degree, wave_degree,
+    lds_queue, hub_cap)."""
     table = {}
-    for h in hubs:
-        for w in waves:
+
+    def time(h, w, q, c):
+        if (h, w, q, c) not in table:
             eng.set_tuning(h, w)
+            eng.set_queues(q, c)
             best = math.inf
             for _ in range(runs):
                 best = min(best, eng.run(roots, policy, variant, download=False).total_ms)
-            table[(h, w)] = best
-    params = min(table, key=table.get)
-    eng.set_tuning(*params)
-    return Tuned(params, table[params], {f"{h}/{w}": v for (h, w), v in table.items()})
+            table[(h, w, q, c)] = best
+        return table[(h, w, q, c)]
+
+    q0, c0 = eng.queues
+    h, w = min(((h, w) for h in hubs for w in waves), key=lambda hw: time(*hw, q0, c0))
+    q = min(lds_queues, key=lambda x: time(h, w, x, c0))
+    c = min(hub_caps, key=lambda x: time(h, w, q, x))
+    params = (h, w, q, c)
+    eng.set_tuning(h, w)
+    eng.set_queues(q, c)
+    return Tuned(params, table[params], {"/".join(map(str, k)): v for k, v in table.items()})
 
 
 def autotune_msbfs(ms: MultiSourceBFS, sources, runs: int = 2, waves=WAVE_CANDIDATES + (256, 1 << 30),

@@ -37,6 +37,8 @@ FEATURE_NAMES = ("log2_frontier_fraction", "log2_frontier_edge_fraction", "log2_
 UNVISITED = 2147483647
 DEFAULT_HUB_DEGREE = 4096
 DEFAULT_WAVE_DEGREE = 8
+DEFAULT_LDS_QUEUE = 2048
+DEFAULT_HUB_CAP = 4096
 
 
 def variant_id(v) -> int:
@@ -107,6 +109,7 @@ class BFSEngine:
         self._choices = np.zeros(self.max_levels, dtype=np.int32)
         self.model_outputs = 0
         self.tuning = (DEFAULT_HUB_DEGREE, DEFAULT_WAVE_DEGREE)
+        self.queues = (DEFAULT_LDS_QUEUE, DEFAULT_HUB_CAP)
 
     # ------------------------------------------------------------ set-up
 
@@ -130,6 +133,12 @@ class BFSEngine:
     def set_tuning(self, hub_degree: int = DEFAULT_HUB_DEGREE, wave_degree: int = DEFAULT_WAVE_DEGREE) -> None:
         N.check_hip(N.hip_lib().bw_bfs_engine_set_tuning(self._e, int(hub_degree), int(wave_degree)))
         self.tuning = (int(hub_degree), int(wave_degree))
+
+    def set_queues(self, lds_queue: int = DEFAULT_LDS_QUEUE, hub_cap: int = DEFAULT_HUB_CAP) -> None:
+        """Discoveries staged per workgroup in LDS and hub-queue capacity
+        (bw_bfs_engine_set_queues); depths never depend on either."""
+        N.check_hip(N.hip_lib().bw_bfs_engine_set_queues(self._e, int(lds_queue), int(hub_cap)))
+        self.queues = (int(lds_queue), int(hub_cap))
 
     # -------------------------------------------------------------- runs
 

@@ -40,8 +40,11 @@ namespace bw {
 namespace {
 
 constexpr int kBlock = 256;
-constexpr uint32_t kLdsQueue = 2048;   // staged discoveries per workgroup
-constexpr uint32_t kHubCap = 4096;     // hubs per level (more go to the normal queue)
+// Staged discoveries per workgroup (dynamic LDS of the level kernel) and
+// hubs per level (more go to the normal queue): runtime sizes up to these
+// maxima (bw_bfs_engine_set_queues).
+constexpr uint32_t kLdsQueueMax = BW_BFS_MAX_LDS_QUEUE;
+constexpr uint32_t kHubCapMax = BW_BFS_MAX_HUB_CAP;
 constexpr uint32_t kParentScan = 32;   // in-edges scanned for a vertex's likely parent
 constexpr uint32_t kNoParent = 0xFFFFFFFFu;
 constexpr int kFeatures = BW_BFS_FEATURES;
@@ -60,7 +63,7 @@ struct BfsState {
     uint32_t qcount;    // normal frontier queue size
     uint32_t hubcount;  // hub frontier queue size
     uint32_t qout;      // appended to the next normal queue during the level
-    uint32_t hubout;    // appended to the next hub queue (may exceed kHubCap)
+    uint32_t hubout;    // appended to the next hub queue (may exceed the hub capacity)
     int32_t choice;     // variant of this level
     uint32_t num_levels;
     int32_t prev_choice;        // variant of the previous level, -1 at level 0
@@ -106,7 +109,7 @@ struct Buffers {
     int32_t *levels;
     uint32_t *q[2];
     uint32_t *hubq[2];
-    uint64_t *hub_prefix;  // kHubCap + 1
+    uint64_t *hub_prefix;  // hub_cap + 1
     uint32_t *bits[3];     // frontier bitmaps, level d reads bits[d % 3]
     uint32_t *vis;         // visited bitmap
     const uint32_t *parent;  // per vertex: its likely parent (in-neighbour of largest out-degree among
@@ -121,6 +124,8 @@ struct Buffers {
     float ns_per_tick;
     uint32_t hub_degree;   // out-degree above which a discovered vertex is a hub
     uint32_t wave_degree;  // in-degree above which BU_WAVE probes with a whole wave
+    uint32_t lds_queue;    // staged discoveries per workgroup (<= kLdsQueueMax)
+    uint32_t hub_cap;      // hub queue capacity, a multiple of kBlock (<= kHubCapMax)
 };
 
 struct PolicyArgs {
@@ -141,6 +146,7 @@ struct Emitter {
     uint32_t *lbuf;
     uint32_t *lcount;
     uint32_t *lbase;
+    uint32_t cap;  // lbuf's size
     uint64_t outsum = 0;
     double sq = 0.0;
 
@@ -158,9 +164,9 @@ struct Emitter {
         if (mark) atomicOr(&b.bits[(d + 1) % 3][w >> 5], 1u << (w & 31));
         if (dw > b.hub_degree) {
             // Slots are handed out by a counter that only grows, so every
-            // slot below min(count, kHubCap) is written exactly once.
+            // slot below min(count, hub_cap) is written exactly once.
             const uint32_t h = atomicAdd(&b.st->hubout, 1u);
-            if (h < kHubCap) {
+            if (h < b.hub_cap) {
                 b.hubq[nq][h] = w;
                 return;
             }
@@ -174,7 +180,7 @@ struct Emitter {
         if (lane == leader) base = atomicAdd(lcount, n);
         base = __shfl(base, leader, kWave);
         const uint32_t pos = base + rank;
-        if (pos < kLdsQueue) {
+        if (pos < cap) {
             lbuf[pos] = w;
         } else {
             b.q[nq][atomicAdd(&b.st->qout, 1u)] = w;
@@ -186,9 +192,9 @@ struct Emitter {
     __device__ __forceinline__ void flush(const Buffers &b, uint32_t d, uint32_t reserve)
     {
         __syncthreads();
-        const uint32_t n = min(*lcount, kLdsQueue);
+        const uint32_t n = min(*lcount, cap);
         __syncthreads();
-        if (n == 0 || n + reserve <= kLdsQueue) return;
+        if (n == 0 || n + reserve <= cap) return;
         if (threadIdx.x == 0) *lbase = atomicAdd(&b.st->qout, n);
         __syncthreads();
         const uint32_t g0 = *lbase;
@@ -517,7 +523,7 @@ __device__ void level_ref_pull_warp(const Graph &g, const Buffers &b, uint32_t d
 
 __global__ __launch_bounds__(kBlock) void bfs_engine_step(Graph g, Buffers b)
 {
-    __shared__ uint32_t lbuf[kLdsQueue];
+    extern __shared__ uint32_t lbuf[];  // b.lds_queue entries
     __shared__ uint32_t lcount, lbase;
     __shared__ uint64_t stotal;
     __shared__ uint32_t sstart[kBlock];
@@ -537,7 +543,7 @@ __global__ __launch_bounds__(kBlock) void bfs_engine_step(Graph g, Buffers b)
 
     if (threadIdx.x == 0) lcount = 0;
     __syncthreads();
-    Emitter em{lbuf, &lcount, &lbase};
+    Emitter em{lbuf, &lcount, &lbase, b.lds_queue};
     const bool fill = st.fill_levels != 0;
     if (variant == BU || variant == BU_WAVE || variant == kRef + BW_BFS_VERTEX_PULL) {
         // Bottom-up forms write the deferred depths in their vertex loop.
@@ -560,7 +566,7 @@ __global__ __launch_bounds__(kBlock) void bfs_engine_step(Graph g, Buffers b)
         default: level_td(g, b, d, em, sstart, sscan, &stotal);
         }
     }
-    em.flush(b, d, kLdsQueue);  // reserve everything: moves all that is staged
+    em.flush(b, d, em.cap);  // reserve everything: moves all that is staged
 
     // Workgroup partials of the discovered vertices' out-degrees.
     uint64_t os = em.outsum;
@@ -821,22 +827,20 @@ __global__ __launch_bounds__(kBlock) void bfs_engine_epilogue(Graph g, Buffers b
         rsq[wid] = sq;
     }
     const uint32_t d = st.depth;
-    const uint32_t nhub = min(st.hubout, kHubCap);
+    const uint32_t nhub = min(st.hubout, b.hub_cap);
     const uint32_t nnew = nhub + st.qout;
     const bool deferred = defer_depths(st, b, g.V);  // this level's, before the state moves on
     __syncthreads();
 
-    // Hub degree prefix of the next frontier (kHubCap / kBlock hubs per
-    // thread), read by the next top-down level.
+    // Hub degree prefix of the next frontier (hub_cap / kBlock consecutive
+    // hubs per thread), read by the next top-down level.
     {
-        constexpr uint32_t kPer = kHubCap / kBlock;
+        const uint32_t per = b.hub_cap / kBlock;
         const uint32_t nq = (d & 1) ^ 1;
-        uint64_t mine[kPer];
         uint64_t local = 0;
-        for (uint32_t k = 0; k < kPer; k++) {
-            const uint32_t h = tid * kPer + k;
-            mine[k] = h < nhub ? out_degree(g, b.hubq[nq][h]) : 0;
-            local += mine[k];
+        for (uint32_t k = 0; k < per; k++) {
+            const uint32_t h = tid * per + k;
+            local += h < nhub ? out_degree(g, b.hubq[nq][h]) : 0;
         }
         // Exclusive scan of the threads' sums: within each wave by
         // shuffles, then the preceding waves' totals.
@@ -849,12 +853,12 @@ __global__ __launch_bounds__(kBlock) void bfs_engine_epilogue(Graph g, Buffers b
         __syncthreads();
         uint64_t run = incl - local;
         for (int w = 0; w < wid; w++) run += hscan[w];
-        for (uint32_t k = 0; k < kPer; k++) {
-            const uint32_t h = tid * kPer + k;
+        for (uint32_t k = 0; k < per; k++) {
+            const uint32_t h = tid * per + k;
             if (h <= nhub) b.hub_prefix[h] = run;
-            run += mine[k];
+            run += h < nhub ? out_degree(g, b.hubq[nq][h]) : 0;
         }
-        if (tid == kBlock - 1 && nhub == kHubCap) b.hub_prefix[kHubCap] = run;
+        if (tid == kBlock - 1 && nhub == b.hub_cap) b.hub_prefix[b.hub_cap] = run;
     }
 
     if (tid == 0) {
@@ -917,7 +921,7 @@ __global__ __launch_bounds__(kBlock) void bfs_engine_init(Graph g, Buffers b, Po
             visited++;
             edges += dr;
             sq += static_cast<double>(dr) * dr;
-            if (dr > b.hub_degree && nh < kHubCap) {
+            if (dr > b.hub_degree && nh < b.hub_cap) {
                 b.hubq[0][nh] = r;
                 b.hub_prefix[nh + 1] = b.hub_prefix[nh] + dr;
                 nh++;
@@ -1070,7 +1074,8 @@ int check_policy(const bw_bfs_engine *e, int32_t policy, int32_t fixed, int32_t 
 
 int launch_level(bw_bfs_engine *e, const PolicyArgs &p, hipStream_t s)
 {
-    hipLaunchKernelGGL(bfs_engine_step, dim3(e->grid), dim3(kBlock), 0, s, e->g, e->b);
+    hipLaunchKernelGGL(bfs_engine_step, dim3(e->grid), dim3(kBlock), sizeof(uint32_t) * e->b.lds_queue, s, e->g,
+                       e->b);
     BW_LAUNCH_CHECK();
     hipLaunchKernelGGL(bfs_engine_epilogue, dim3(1), dim3(kBlock), 0, s, e->g, e->b, p, e->grid);
     BW_LAUNCH_CHECK();
@@ -1109,9 +1114,9 @@ extern "C" int bw_bfs_engine_create(bw_bfs_engine **out, const bw_graph_dev *g, 
     e->b.levels = levels;
     alloc(&e->b.q[0], sizeof(uint32_t) * V);
     alloc(&e->b.q[1], sizeof(uint32_t) * V);
-    alloc(&e->b.hubq[0], sizeof(uint32_t) * kHubCap);
-    alloc(&e->b.hubq[1], sizeof(uint32_t) * kHubCap);
-    alloc(&e->b.hub_prefix, sizeof(uint64_t) * (kHubCap + 1));
+    alloc(&e->b.hubq[0], sizeof(uint32_t) * kHubCapMax);
+    alloc(&e->b.hubq[1], sizeof(uint32_t) * kHubCapMax);
+    alloc(&e->b.hub_prefix, sizeof(uint64_t) * (kHubCapMax + 1));
     for (auto &bm : e->b.bits) alloc(&bm, sizeof(uint32_t) * words);
     alloc(&e->b.vis, sizeof(uint32_t) * words);
     alloc(&e->no_in, sizeof(uint32_t) * words);
@@ -1126,6 +1131,8 @@ extern "C" int bw_bfs_engine_create(bw_bfs_engine **out, const bw_graph_dev *g, 
     e->b.ns_per_tick = 1e6f / static_cast<float>(rate_khz);
     e->b.hub_degree = BW_BFS_DEFAULT_HUB_DEGREE;
     e->b.wave_degree = BW_BFS_DEFAULT_WAVE_DEGREE;
+    e->b.lds_queue = BW_BFS_DEFAULT_LDS_QUEUE;
+    e->b.hub_cap = BW_BFS_DEFAULT_HUB_CAP;
     if (!ok) {
         delete e;
         return fail(BW_HIP_ERR_NOMEM, "device allocation of the BFS engine failed");
@@ -1190,6 +1197,20 @@ extern "C" int bw_bfs_engine_set_tuning(bw_bfs_engine *e, uint32_t hub_degree, u
     e->drop_exec();  // the captured batch holds the old thresholds
     e->b.hub_degree = hub_degree;
     e->b.wave_degree = wave_degree;
+    return BW_HIP_OK;
+}
+
+extern "C" int bw_bfs_engine_set_queues(bw_bfs_engine *e, uint32_t lds_queue, uint32_t hub_cap)
+{
+    if (!e) return fail(BW_HIP_ERR_ARG, "engine is NULL");
+    if (lds_queue < kBlock || lds_queue > kLdsQueueMax || lds_queue % kWave)
+        return fail(BW_HIP_ERR_ARG, "lds_queue must be a multiple of 64 in [256, " + std::to_string(kLdsQueueMax) +
+                                        "]");
+    if (hub_cap < kBlock || hub_cap > kHubCapMax || hub_cap % kBlock)
+        return fail(BW_HIP_ERR_ARG, "hub_cap must be a multiple of 256 in [256, " + std::to_string(kHubCapMax) + "]");
+    e->drop_exec();  // the captured batch holds the old sizes
+    e->b.lds_queue = lds_queue;
+    e->b.hub_cap = hub_cap;
     return BW_HIP_OK;
 }
 
@@ -1301,16 +1322,16 @@ extern "C" int bw_bfs_engine_snapshot(bw_bfs_engine *e, int32_t restore, void *s
         if (restore) return fail(BW_HIP_ERR_ARG, "no snapshot to restore");
         BW_HIP_CHECK(hipMalloc(&e->snap_levels, sizeof(int32_t) * V));
         BW_HIP_CHECK(hipMalloc(&e->snap_q, sizeof(uint32_t) * V));
-        BW_HIP_CHECK(hipMalloc(&e->snap_hub, sizeof(uint32_t) * kHubCap));
+        BW_HIP_CHECK(hipMalloc(&e->snap_hub, sizeof(uint32_t) * kHubCapMax));
         BW_HIP_CHECK(hipMalloc(&e->snap_bits, sizeof(uint32_t) * 4 * words));
-        BW_HIP_CHECK(hipMalloc(&e->snap_prefix, sizeof(uint64_t) * (kHubCap + 1)));
+        BW_HIP_CHECK(hipMalloc(&e->snap_prefix, sizeof(uint64_t) * (kHubCapMax + 1)));
         BW_HIP_CHECK(hipMalloc(&e->snap_st, sizeof(BfsState)));
     }
     BfsState st;
     BW_HIP_CHECK(hipMemcpyAsync(&st, restore ? e->snap_st : e->b.st, sizeof st, hipMemcpyDeviceToHost, s));
     BW_HIP_CHECK(hipStreamSynchronize(s));
     const uint32_t cur = st.depth & 1;
-    const size_t nq = std::max<size_t>(st.qcount, 1), nh = std::max<size_t>(std::min(st.hubcount, kHubCap), 1);
+    const size_t nq = std::max<size_t>(st.qcount, 1), nh = std::max<size_t>(std::min(st.hubcount, e->b.hub_cap), 1);
     auto copy = [&](void *saved, void *live, size_t bytes) {
         return restore ? hipMemcpyAsync(live, saved, bytes, hipMemcpyDeviceToDevice, s)
                        : hipMemcpyAsync(saved, live, bytes, hipMemcpyDeviceToDevice, s);
@@ -1318,7 +1339,7 @@ extern "C" int bw_bfs_engine_snapshot(bw_bfs_engine *e, int32_t restore, void *s
     BW_HIP_CHECK(copy(e->snap_levels, e->b.levels, sizeof(int32_t) * V));
     BW_HIP_CHECK(copy(e->snap_q, e->b.q[cur], sizeof(uint32_t) * nq));
     BW_HIP_CHECK(copy(e->snap_hub, e->b.hubq[cur], sizeof(uint32_t) * nh));
-    BW_HIP_CHECK(copy(e->snap_prefix, e->b.hub_prefix, sizeof(uint64_t) * (kHubCap + 1)));
+    BW_HIP_CHECK(copy(e->snap_prefix, e->b.hub_prefix, sizeof(uint64_t) * (e->b.hub_cap + 1)));
     for (int i = 0; i < 3; i++) BW_HIP_CHECK(copy(e->snap_bits + i * words, e->b.bits[i], sizeof(uint32_t) * words));
     BW_HIP_CHECK(copy(e->snap_bits + 3 * words, e->b.vis, sizeof(uint32_t) * words));
     if (restore) hipLaunchKernelGGL(copy_state, dim3(1), dim3(1), 0, s, e->b.st, e->snap_st);
@@ -1346,7 +1367,8 @@ extern "C" int bw_bfs_engine_step_forced(bw_bfs_engine *e, int32_t variant, floa
     int st = BW_HIP_OK;
     if (hipEventRecord(ev0, s) != hipSuccess) st = fail(BW_HIP_ERR_RUNTIME, "event record failed");
     if (st == BW_HIP_OK) {
-        hipLaunchKernelGGL(bfs_engine_step, dim3(e->grid), dim3(kBlock), 0, s, e->g, e->b);
+        hipLaunchKernelGGL(bfs_engine_step, dim3(e->grid), dim3(kBlock), sizeof(uint32_t) * e->b.lds_queue, s,
+                           e->g, e->b);
         hipError_t le = hipGetLastError();
         if (le != hipSuccess) st = hip_fail(le, "bfs_engine_step", __FILE__, __LINE__);
     }

@@ -256,7 +256,8 @@ def simulate(samples: Samples, choices: np.ndarray) -> dict:
 
 
 def evaluate(engine: BFSEngine, g, sources, model: Model, repeats: int = 1, runs: int = 3,
-             levels_per_sync: int = 8, samples_out=None, reference=None, slow_factor: float = 20.0) -> dict:
+             levels_per_sync: int = 8, samples_out=None, reference=None, slow_factor: float = 20.0,
+             exhaustive_sources: int | None = None, progress=None) -> dict:
     """Learned switch vs best fixed vs per-level oracle vs Beamer's rule on
     `sources`, over every variant the engine can run.
 
@@ -268,9 +269,16 @@ def evaluate(engine: BFSEngine, g, sources, model: Model, repeats: int = 1, runs
     learned policy evaluates the model on the device at every level.  A
     fixed variant slower than `slow_factor` times the best in the
     simulation is not executed (its executed time is reported as None).
-    `reference` ({source: depths} from an independent BFS) checks the
-    depths of the exhaustive, learned and Beamer runs of every source."""
-    samples, results = collect(engine, sources, repeats=repeats)
+    The simulation uses the first `exhaustive_sources` sources (all by
+    default), the executed runs every source.  `reference` ({source:
+    depths} from an independent BFS) checks the depths of the exhaustive,
+    learned and Beamer runs of every source; `progress(message)` is called
+    as each source is done."""
+    sim_src = list(sources)[:exhaustive_sources] if exhaustive_sources else list(sources)
+    say = progress or (lambda msg: None)
+    samples, results = collect(engine, sim_src, repeats=repeats,
+                               progress=lambda i, v, n: say(f"exhaustive source {i + 1}/{len(sim_src)} ({v}): "
+                                                            f"{n} levels"))
     if samples_out:
         samples.save(samples_out)
     K = len(VARIANTS)
@@ -293,11 +301,12 @@ def evaluate(engine: BFSEngine, g, sources, model: Model, repeats: int = 1, runs
     edges = 0
     mismatches = ref_mismatches = beamer_mismatches = 0
     beamer_hist = np.zeros(K, dtype=np.int64)
-    for s in sources:
-        levels_ref, counts_ref = results[int(s)]
-        if reference is not None and not np.array_equal(levels_ref, reference[int(s)]):
-            ref_mismatches += 1
-        edges += teps_edges(g, levels_ref)
+    for i, s in enumerate(sources):
+        exhausted = int(s) in results
+        if exhausted:
+            levels_ref, counts_ref = results[int(s)]
+            if reference is not None and not np.array_equal(levels_ref, reference[int(s)]):
+                ref_mismatches += 1
         for k in np.flatnonzero(run_fixed):
             # A variant far off the best in the simulation is run once only.
             reps = runs if fixed_tot[k] <= 2.0 * fixed_tot[best_fixed] else 1
@@ -312,8 +321,13 @@ def evaluate(engine: BFSEngine, g, sources, model: Model, repeats: int = 1, runs
             best = min(best, r.total_ms)
         exec_learned += best
         lv = engine.levels()
-        if not (np.array_equal(lv, levels_ref) and np.array_equal(r.counts, counts_ref)):
-            mismatches += 1
+        e = teps_edges(g, lv)
+        edges += e
+        if not exhausted:
+            levels_ref, counts_ref = lv, r.counts
+        else:
+            if not (np.array_equal(lv, levels_ref) and np.array_equal(r.counts, counts_ref)):
+                mismatches += 1
         if reference is not None and not np.array_equal(lv, reference[int(s)]):
             ref_mismatches += 1
         best = np.inf
@@ -324,16 +338,20 @@ def evaluate(engine: BFSEngine, g, sources, model: Model, repeats: int = 1, runs
         lv = engine.levels()
         if not (np.array_equal(lv, levels_ref) and np.array_equal(rb.counts, counts_ref)):
             beamer_mismatches += 1
-        m = samples.source == int(s)
-        t = samples.ms[m]
-        sim_beamer += float(t[np.arange(t.shape[0]), rb.choices[:t.shape[0]]].sum())
+        if exhausted:
+            m = samples.source == int(s)
+            t = samples.ms[m]
+            sim_beamer += float(t[np.arange(t.shape[0]), rb.choices[:t.shape[0]]].sum())
         beamer_hist += np.bincount(rb.choices, minlength=K)
+        say(f"executed source {i + 1}/{len(sources)} ({int(s)}): learned {exec_learned:.3f} ms, "
+            f"beamer {exec_beamer:.3f} ms so far")
     exec_fixed = np.where(run_fixed, exec_fixed, np.inf)
     exec_best = int(np.argmin(exec_fixed))
     gteps = lambda ms: edges / (ms * 1e-3) / 1e9 if ms > 0 else 0.0  # noqa: E731
     fin = lambda x: float(x) if np.isfinite(x) else None  # noqa: E731
     return {
         "sources": [int(s) for s in sources],
+        "exhaustive_sources": [int(s) for s in sim_src],
         "levels": int(samples.X.shape[0]),
         "teps_edges": int(edges),
         "variants": list(engine.available),

@@ -171,15 +171,20 @@ def bfs_cpu_leg(g, sources, teps_div: int):
     threads, allowed, ncpu = host_cpus()
     deg = np.diff(g.offsets()).astype(np.int64)
     depths, edges, secs = {}, 0, 0.0
-    for s in sources:
+    for i, s in enumerate(sources):
         lv, _, sec = O.bfs_vertex_push_omp(csr, int(s), threads, timed=True)
         depths[int(s)] = lv
         edges += int(deg[lv != O.UNVISITED].sum()) // teps_div
         secs += sec
+        log(f"cpu bfs source {i + 1}/{len(sources)} ({int(s)}) on {threads} threads: {sec:.3f} s")
     s0 = int(sources[0])
+
+    def one():
+        res = O.bfs_vertex_push_omp(csr, s0, 1, timed=True)[::2]
+        log(f"cpu bfs source {s0} on one pinned core: {res[1]:.3f} s")
+        return res
     with OneCpu() as pin:
-        _, sec1, secs1 = median_seconds(lambda: O.bfs_vertex_push_omp(csr, s0, 1, timed=True)[::2], runs=3,
-                                        warmup=0)
+        _, sec1, secs1 = median_seconds(one, runs=3, warmup=0)
     e0 = int(deg[depths[s0] != O.UNVISITED].sum()) // teps_div
     conv = "reached out-degree sum / 2 (undirected, Graph500)" if teps_div == 2 else "reached out-degree sum"
     base = {"value": round(edges / secs / 1e9, 4), "unit": "GTEPS", "cores": threads, "kind": "port",
@@ -196,9 +201,10 @@ def bfs_learned_switch(scale: int, sources: int, samples_out=None, cpu: bool = T
     sources of out-degree >= 1: the learned per-level switch against the
     best fixed variant of all eleven (native and the reference's kernel
     families), the per-level oracle over the same set and Beamer's
-    direction-optimising rule, simulated on exhaustive per-level timings and
-    executed end to end on the device; every depth checked against the CPU
-    baseline's (the oracle)."""
+    direction-optimising rule, simulated on exhaustive per-level timings
+    (from the first 4 sources: every variant at every level, twice) and
+    executed end to end on the device from every source; every depth
+    checked against the CPU baseline's (the oracle)."""
     from belewitte_amd import bfs_native, device, graph, switch
     t0 = time.perf_counter()
     g = graph.rmat(scale, 16, seed=5, undirected=True)
@@ -217,7 +223,8 @@ def bfs_learned_switch(scale: int, sources: int, samples_out=None, cpu: bool = T
         model, _ = switch.fit(samples)
         origin = "trained in this run on 4 of the sources (no stored model)"
     t0 = time.perf_counter()
-    res = switch.evaluate(eng, g, src, model, repeats=2, runs=3, samples_out=samples_out, reference=ref)
+    res = switch.evaluate(eng, g, src, model, repeats=2, runs=3, samples_out=samples_out, reference=ref,
+                          exhaustive_sources=4, progress=lambda m: log(f"bfs switch: {m}"))
     log(f"bfs switch evaluated in {time.perf_counter() - t0:.1f}s: {res['executed_ms']}")
     # Algorithmic bytes of a top-down pass per source: 8 E_r + 16 V_r.
     deg = np.diff(g.offsets()).astype(np.int64)
@@ -401,7 +408,7 @@ def bfs_batched(cpu: bool = True) -> dict:
     u, base = batch.union(graphs)
     roots = [base[i] + gi.max_out_degree_vertex() for i, gi in enumerate(graphs)]
     dg = device.DeviceGraph(u)
-    eng = bfs_native.BFSEngine(dg)
+    eng = bfs_native.BFSEngine(dg, edge_lists=True)
     eng.set_model(switch.Model.load(switch.MODEL_PATH).w)
     # The best fixed variant, for contrast with the learned per-level switch.
     fixed = {v: min(eng.run(roots, "fixed", v, download=False).total_ms for _ in range(3))
@@ -415,7 +422,9 @@ def bfs_batched(cpu: bool = True) -> dict:
            "policy": "learned per-level switch (shipped model)", "choices": r.choices.tolist(),
            "fixed_variant_ms": {k: round(v, 3) for k, v in fixed.items()},
            "vertices": u.vertex_count, "edges": u.edge_count, "reached_edges": edges,
-           "levels": r.num_levels, "tuned_hub_wave": list(tuned.params), "tuning_ms": tuned.table}
+           "levels": r.num_levels,
+           "tuned": dict(zip(("hub_degree", "wave_degree", "lds_queue", "hub_cap"), tuned.params)),
+           "tuning_ms": tuned.table}
     if cpu:
         from concurrent.futures import ThreadPoolExecutor
 

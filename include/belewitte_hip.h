@@ -579,6 +579,19 @@ int bw_bfs_engine_levels(const bw_bfs_engine *engine, int32_t **levels);
 int bw_bfs_engine_set_tuning(bw_bfs_engine *engine, uint32_t hub_degree,
                              uint32_t wave_degree);
 
+/* Queue sizes (the autotuner's other knobs): discoveries each workgroup
+ * stages in LDS before one global append (lds_queue, a multiple of 64 in
+ * [256, BW_BFS_MAX_LDS_QUEUE]; the level kernel's dynamic LDS, so a larger
+ * queue means fewer workgroups per CU), and the hub queue's capacity per
+ * level (hub_cap, a multiple of 256 in [256, BW_BFS_MAX_HUB_CAP]; hubs past
+ * it go to the normal queue).  Depths never depend on either. */
+#define BW_BFS_DEFAULT_LDS_QUEUE 2048
+#define BW_BFS_DEFAULT_HUB_CAP 4096
+#define BW_BFS_MAX_LDS_QUEUE 8192
+#define BW_BFS_MAX_HUB_CAP 16384
+int bw_bfs_engine_set_queues(bw_bfs_engine *engine, uint32_t lds_queue,
+                             uint32_t hub_cap);
+
 /* Most roots of one multi-root search. */
 #define BW_BFS_MAX_ROOTS 16384
 

@@ -338,6 +338,45 @@ def test_tuning_never_changes_depths(cases, hub, wave):
             eng.set_tuning()
 
 
+@pytest.mark.parametrize("lds_queue,hub_cap", [(256, 256), (512, 1024), (8192, 16384)])
+def test_queue_sizes_never_change_depths(cases, lds_queue, hub_cap):
+    """Tiny LDS queues overflow into direct global appends and a tiny hub
+    queue sends most hubs through the normal queue; the largest sizes
+    shrink the level kernel's occupancy.  Depths and counts stay exact."""
+    for g, csr, dg, eng, roots in cases[:3] + cases[5:7]:
+        eng.set_queues(lds_queue, hub_cap)
+        eng.set_tuning(256, 8)  # many hubs
+        try:
+            for v in bfs_native.VARIANTS + ("learned",):
+                eng.set_model(switch.Model.init(np.random.default_rng(4).normal(8, 3, (64, bfs_native.NUM_FEATURES)),
+                                                seed=4, target_mean=10.0).w)
+                r = eng.run(roots[0], "learned" if v == "learned" else "fixed", "td" if v == "learned" else v)
+                want, counts = _want(csr, roots[0])
+                assert np.array_equal(r.levels, want), (g.name, lds_queue, hub_cap, v)
+                assert np.array_equal(r.counts, counts), (g.name, lds_queue, hub_cap, v)
+        finally:
+            eng.set_queues()
+            eng.set_tuning()
+
+
+def test_many_hubs_small_hub_queue(gpu):
+    g = _many_hubs()
+    csr = O.CSR.from_graph(g)
+    dg = device.DeviceGraph(g)
+    eng = bfs_native.BFSEngine(dg)
+    try:
+        want, counts = _want(csr, 0)
+        for q, c in ((256, 256), (8192, 16384)):
+            eng.set_queues(q, c)
+            for v in ("td", "bu-wave", "vertex-push"):
+                r = eng.run(0, "fixed", v)
+                assert np.array_equal(r.levels, want), (q, c, v)
+                assert np.array_equal(r.counts, counts), (q, c, v)
+    finally:
+        eng.close()
+        dg.free()
+
+
 def test_level_limit_is_an_error(gpu):
     g = graph.chain(100)
     dg = device.DeviceGraph(g)
@@ -357,6 +396,10 @@ def test_rejects_bad_arguments(cases):
     fresh = bfs_native.BFSEngine(dg)
     with pytest.raises(N.BelewitteError):
         fresh.run(0, "learned", "td")  # no model set
+    for q, c in ((128, 4096), (8256, 4096), (2050, 4096), (2048, 0), (2048, 300), (2048, 16640)):
+        with pytest.raises(N.BelewitteError):
+            fresh.set_queues(q, c)
+    assert fresh.queues == (bfs_native.DEFAULT_LDS_QUEUE, bfs_native.DEFAULT_HUB_CAP)
     fresh.close()
     n = C.c_uint32()
     assert N.hip_lib().bw_bfs_engine_run(eng._e, 0, 9, 0, 1, 1, 0, None, None, None, C.byref(n), None,

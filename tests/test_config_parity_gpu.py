@@ -11,14 +11,16 @@ is checked against the oracle at the size its figures are measured on:
   and UNIFORM modes, 30 iterations.
 * cfg3: BFS on the Erdos-Renyi graph (2^24 vertices, 2^27 undirected pairs,
   seed 3) and on the 4096 x 4096 grid with 1% of its edges rewired
-  (seed 4), 16 sources each (seed 10): the learned switch, every native
-  variant, the exhaustive per-level collector and the reference's kernel
-  families.
+  (seed 4), 16 sources each (seed 10): the learned switch, Beamer's rule,
+  every engine variant, the exhaustive per-level collector and the
+  reference's kernel families through the reference level loop.
 * cfg4: 64 sources on the directed R-MAT scale 20 (seed 6) in one
   multi-source search, every source checked; 256 R-MAT graphs searched as
-  one disjoint union, every component checked against its own graph.
+  one disjoint union, every component checked against its own graph; both
+  again after the autotuner has chosen their thresholds and queue sizes.
 * cfg5: the learned switch on the undirected R-MAT scale 25 (seed 5) from
-  16 sources (seed 8).
+  16 sources (seed 8), with a fixed native, a fixed reference-family and
+  the Beamer run beside it.
 
 The depth oracle is the textbook queue BFS (oracle_bfs_queue), run for
 several sources at once on the host cores; per-level counts follow from
@@ -153,7 +155,7 @@ def test_cfg3_bfs_full_size(gpu, which):
     src = [int(s) for s in graph.random_sources(g, 16, seed=10)]
     want = oracle_depths(csr, src)
     dg = device.DeviceGraph(g)
-    eng = bfs_native.BFSEngine(dg)
+    eng = bfs_native.BFSEngine(dg, edge_lists=True)
     eng.set_model(switch.Model.load(switch.MODEL_PATH).w)
     try:
         for s, w in zip(src, want):
@@ -161,6 +163,8 @@ def test_cfg3_bfs_full_size(gpu, which):
             r = eng.run(s, "learned", "td")
             assert np.array_equal(r.levels, w), (g.name, s)
             assert np.array_equal(r.counts, level_counts(w)), (g.name, s)
+        print(f"cfg3 {g.name} source {src[0]} beamer", flush=True)
+        assert np.array_equal(eng.run(src[0], "beamer", "td").levels, want[0])
         for k, v in enumerate(bfs_native.VARIANTS):
             s, w = src[k], want[k]
             print(f"cfg3 {g.name} source {s} fixed {v}", flush=True)
@@ -207,15 +211,60 @@ def test_cfg4_multi_source_every_source(gpu):
         dg.free()
 
 
+def test_cfg4_autotuned_every_source_and_component(gpu):
+    """The autotuner at the batch sizes: the 64-source search on R-MAT
+    scale 20 and the 256-graph union, each tuned (thresholds, then queue
+    sizes) and rerun with the chosen parameters; every source and every
+    component checked against the oracle."""
+    g = graph.rmat(20, 16, seed=6)
+    csr = O.CSR.from_graph(g)
+    src = [int(s) for s in graph.random_sources(g, 64, seed=9)]
+    want = oracle_depths(csr, src)
+    dg = device.DeviceGraph(g)
+    ms = bfs_native.MultiSourceBFS(dg)
+    try:
+        tuned = batch.autotune_msbfs(ms, src)
+        print(f"cfg4 64 sources tuned {tuned.params}", flush=True)
+        depths, _, _ = ms.run(src)
+        for i, w in enumerate(want):
+            assert np.array_equal(depths[i], w), (tuned.params, i, src[i])
+    finally:
+        ms.close()
+        dg.free()
+
+    graphs = batch.small_rmat_batch(256, seed=7)
+    u, base = batch.union(graphs)
+    roots = [base[i] + gi.max_out_degree_vertex() for i, gi in enumerate(graphs)]
+    csrs = [O.CSR.from_graph(gi) for gi in graphs]
+    with ThreadPoolExecutor(max_workers=THREADS) as ex:
+        want = list(ex.map(lambda i: O.bfs_queue(csrs[i], graphs[i].max_out_degree_vertex())[0],
+                           range(len(graphs))))
+    dg = device.DeviceGraph(u)
+    eng = bfs_native.BFSEngine(dg, edge_lists=True)
+    eng.set_model(switch.Model.load(switch.MODEL_PATH).w)
+    try:
+        tuned = batch.autotune_engine(eng, roots, "learned", "td")
+        assert (eng.tuning + eng.queues) == tuned.params
+        print(f"cfg4 union tuned {tuned.params}", flush=True)
+        runs = {"learned": eng.run(roots, "learned", "td"), "td": eng.run(roots, "fixed", "td"),
+                "edge-list": eng.run(roots, "fixed", "edge-list")}
+    finally:
+        eng.close()
+        dg.free()
+    for label, r in runs.items():
+        for i, gi in enumerate(graphs):
+            assert np.array_equal(r.levels[base[i]:base[i] + gi.vertex_count], want[i]), (label, i, gi.name)
+
+
 def test_cfg4_union_every_component(gpu):
     graphs = batch.small_rmat_batch(256, seed=7)
     u, base = batch.union(graphs)
     roots = [base[i] + gi.max_out_degree_vertex() for i, gi in enumerate(graphs)]
     dg = device.DeviceGraph(u)
-    eng = bfs_native.BFSEngine(dg)
+    eng = bfs_native.BFSEngine(dg, edge_lists=True)
     eng.set_model(switch.Model.load(switch.MODEL_PATH).w)
     try:
-        runs = {"learned": eng.run(roots, "learned", "td")}
+        runs = {"learned": eng.run(roots, "learned", "td"), "beamer": eng.run(roots, "beamer", "td")}
         for v in bfs_native.VARIANTS:
             runs[v] = eng.run(roots, "fixed", v)
     finally:
@@ -240,7 +289,7 @@ def test_cfg5_learned_switch_scale_25(gpu):
     src = [int(s) for s in graph.random_sources(g, 16, seed=8)]
     want = oracle_depths(csr, src)
     dg = device.DeviceGraph(g)
-    eng = bfs_native.BFSEngine(dg)
+    eng = bfs_native.BFSEngine(dg, edge_lists=True)
     eng.set_model(switch.Model.load(switch.MODEL_PATH).w)
     try:
         for s, w in zip(src, want):
@@ -248,9 +297,11 @@ def test_cfg5_learned_switch_scale_25(gpu):
             r = eng.run(s, "learned", "td")
             assert np.array_equal(r.levels, w), s
             assert np.array_equal(r.counts, level_counts(w)), s
-        for v, s, w in (("td", src[0], want[0]), ("bu-wave", src[1], want[1])):
+        for v, s, w in (("td", src[0], want[0]), ("bu-wave", src[1], want[1]), ("edge-list", src[2], want[2])):
             print(f"cfg5 {g.name} source {s} fixed {v}", flush=True)
             assert np.array_equal(eng.run(s, "fixed", v).levels, w), (v, s)
+        print(f"cfg5 {g.name} source {src[3]} beamer", flush=True)
+        assert np.array_equal(eng.run(src[3], "beamer", "td").levels, want[3])
     finally:
         eng.close()
         dg.free()
