@@ -308,8 +308,8 @@ __device__ void level_td(const Graph &g, const Buffers &b, uint32_t d, Emitter &
     }
 }
 
-__device__ void level_bu(const Graph &g, const Buffers &b, uint32_t d, Emitter &em, bool wave, bool fill,
-                         bool use_parent = true)
+template<bool USE_PARENT>
+__device__ void level_bu(const Graph &g, const Buffers &b, uint32_t d, Emitter &em, bool wave, bool fill)
 {
     const int32_t nd = static_cast<int32_t>(d) + 1;
     const uint32_t *front = b.bits[d % 3];
@@ -325,7 +325,7 @@ __device__ void level_bu(const Graph &g, const Buffers &b, uint32_t d, Emitter &
         const bool unvisited = v64 < V && !test_bit(b.vis, v);
         // The likely parent first: one coalesced load and one probe resolve
         // most vertices of a big level; the in-edges are scanned otherwise.
-        const uint32_t pv = unvisited && use_parent ? b.parent[v] : kNoParent;
+        const uint32_t pv = USE_PARENT && unvisited ? b.parent[v] : kNoParent;
         const bool quick = pv != kNoParent && test_bit(front, pv);
         uint32_t s = 0, e = 0;
         if (unvisited && !quick) {
@@ -521,6 +521,12 @@ __device__ void level_ref_pull_warp(const Graph &g, const Buffers &b, uint32_t d
 
 // ---------------------------------------------------------- level kernel
 
+// The level kernel in two instantiations: REF = false runs the native
+// variants, REF = true the reference's families; each returns at once when
+// the level's variant is the other kind.  Kept apart so that the native
+// kernel's code is compiled on its own (with the reference families in the
+// same kernel its levels ran about 1.5x slower on MI355X).
+template<bool REF>
 __global__ __launch_bounds__(kBlock) void bfs_engine_step(Graph g, Buffers b)
 {
     extern __shared__ uint32_t lbuf[];  // b.lds_queue entries
@@ -535,6 +541,7 @@ __global__ __launch_bounds__(kBlock) void bfs_engine_step(Graph g, Buffers b)
     if (st.done) return;
     const uint32_t d = st.depth;
     const int32_t variant = st.choice;
+    if ((variant >= kRef) != REF) return;
     if (blockIdx.x == 0 && threadIdx.x == 0) st.t_step = wall_clock64();
 
     // Clear the bitmap that the next level will write.
@@ -545,25 +552,36 @@ __global__ __launch_bounds__(kBlock) void bfs_engine_step(Graph g, Buffers b)
     __syncthreads();
     Emitter em{lbuf, &lcount, &lbase, b.lds_queue};
     const bool fill = st.fill_levels != 0;
-    if (variant == BU || variant == BU_WAVE || variant == kRef + BW_BFS_VERTEX_PULL) {
-        // Bottom-up forms write the deferred depths in their vertex loop.
-        level_bu(g, b, d, em, variant == BU_WAVE, fill, variant != kRef + BW_BFS_VERTEX_PULL);
-    } else {
-        if (fill) {  // the previous level's depths, from this level's frontier bitmap
-            const uint32_t *front = b.bits[d % 3];
-            for (uint64_t v = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x; v < g.V;
-                 v += static_cast<uint64_t>(gridDim.x) * kBlock)
-                if (test_bit(front, static_cast<uint32_t>(v))) b.levels[v] = static_cast<int32_t>(d);
+    // Bottom-up forms write the deferred depths in their vertex loop; the
+    // others first write them from this level's frontier bitmap.
+    auto fill_depths = [&] {
+        if (!fill) return;
+        const uint32_t *front = b.bits[d % 3];
+        for (uint64_t v = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x; v < g.V;
+             v += static_cast<uint64_t>(gridDim.x) * kBlock)
+            if (test_bit(front, static_cast<uint32_t>(v))) b.levels[v] = static_cast<int32_t>(d);
+    };
+    if constexpr (!REF) {
+        if (variant == TD) {
+            fill_depths();
+            level_td(g, b, d, em, sstart, sscan, &stotal);
+        } else {
+            level_bu<true>(g, b, d, em, variant == BU_WAVE, fill);
         }
-        switch (variant - kRef) {
-        case BW_BFS_EDGE_LIST: level_ref_edges<BW_BFS_EDGE_LIST>(g, b, d, em); break;
-        case BW_BFS_REV_EDGE_LIST: level_ref_edges<BW_BFS_REV_EDGE_LIST>(g, b, d, em); break;
-        case BW_BFS_STRUCT_EDGE_LIST: level_ref_edges<BW_BFS_STRUCT_EDGE_LIST>(g, b, d, em); break;
-        case BW_BFS_REV_STRUCT_EDGE_LIST: level_ref_edges<BW_BFS_REV_STRUCT_EDGE_LIST>(g, b, d, em); break;
-        case BW_BFS_VERTEX_PUSH: level_ref_push(g, b, d, em); break;
-        case BW_BFS_VERTEX_PUSH_WARP: level_ref_push_warp(g, b, d, em); break;
-        case BW_BFS_VERTEX_PULL_WARP: level_ref_pull_warp(g, b, d, em); break;
-        default: level_td(g, b, d, em, sstart, sscan, &stotal);
+    } else {
+        if (variant == kRef + BW_BFS_VERTEX_PULL) {
+            level_bu<false>(g, b, d, em, false, fill);
+        } else {
+            fill_depths();
+            switch (variant - kRef) {
+            case BW_BFS_EDGE_LIST: level_ref_edges<BW_BFS_EDGE_LIST>(g, b, d, em); break;
+            case BW_BFS_REV_EDGE_LIST: level_ref_edges<BW_BFS_REV_EDGE_LIST>(g, b, d, em); break;
+            case BW_BFS_STRUCT_EDGE_LIST: level_ref_edges<BW_BFS_STRUCT_EDGE_LIST>(g, b, d, em); break;
+            case BW_BFS_REV_STRUCT_EDGE_LIST: level_ref_edges<BW_BFS_REV_STRUCT_EDGE_LIST>(g, b, d, em); break;
+            case BW_BFS_VERTEX_PUSH: level_ref_push(g, b, d, em); break;
+            case BW_BFS_VERTEX_PUSH_WARP: level_ref_push_warp(g, b, d, em); break;
+            default: level_ref_pull_warp(g, b, d, em); break;
+            }
         }
     }
     em.flush(b, d, em.cap);  // reserve everything: moves all that is staged
@@ -1012,6 +1030,7 @@ struct bw_bfs_engine {
     int32_t mlp_out = 0;
     int32_t *schedule = nullptr;
     int32_t schedule_cap = 0;
+    uint32_t schedule_kinds = 1;  // level kernels the schedule needs (see policy_kinds)
     uint32_t *roots = nullptr;
     uint32_t roots_cap = 0;
     uint32_t *no_in = nullptr;  // vertices without in-edges (bitmap)
@@ -1072,11 +1091,34 @@ int check_policy(const bw_bfs_engine *e, int32_t policy, int32_t fixed, int32_t 
     return BW_HIP_OK;
 }
 
+// Which level kernels a policy may need: bit 0 the native one, bit 1 the
+// reference families' (a level launches each it may need; the one whose
+// kind the level's variant is not returns at once).
+uint32_t policy_kinds(const bw_bfs_engine *e, const PolicyArgs &p)
+{
+    switch (p.policy) {
+    case FIXED: return p.fixed < kRef ? 1u : 2u;
+    case SCHEDULE: return e->schedule_kinds;
+    case LEARNED: return p.nout > kRef ? 3u : 1u;
+    case BEAMER: return 1u;
+    default: return 3u;  // HOST: the variant is set on the device
+    }
+}
+
+int launch_step(bw_bfs_engine *e, bool ref, hipStream_t s)
+{
+    const size_t lds = sizeof(uint32_t) * e->b.lds_queue;
+    if (ref) hipLaunchKernelGGL(bfs_engine_step<true>, dim3(e->grid), dim3(kBlock), lds, s, e->g, e->b);
+    else hipLaunchKernelGGL(bfs_engine_step<false>, dim3(e->grid), dim3(kBlock), lds, s, e->g, e->b);
+    BW_LAUNCH_CHECK();
+    return BW_HIP_OK;
+}
+
 int launch_level(bw_bfs_engine *e, const PolicyArgs &p, hipStream_t s)
 {
-    hipLaunchKernelGGL(bfs_engine_step, dim3(e->grid), dim3(kBlock), sizeof(uint32_t) * e->b.lds_queue, s, e->g,
-                       e->b);
-    BW_LAUNCH_CHECK();
+    const uint32_t kinds = policy_kinds(e, p);
+    if (kinds & 1u) BW_TRY(launch_step(e, false, s));
+    if (kinds & 2u) BW_TRY(launch_step(e, true, s));
     hipLaunchKernelGGL(bfs_engine_epilogue, dim3(1), dim3(kBlock), 0, s, e->g, e->b, p, e->grid);
     BW_LAUNCH_CHECK();
     return BW_HIP_OK;
@@ -1169,6 +1211,7 @@ extern "C" int bw_bfs_engine_set_model(bw_bfs_engine *e, const float *weights, i
         return fail(BW_HIP_ERR_ARG, "bw_bfs_engine_set_model: the model needs one output per native variant, or "
                                     "per engine variant");
     if (!e->mlp) BW_HIP_CHECK(hipMalloc(&e->mlp, sizeof(float) * mlp_len(kMaxOut)));
+    if (outputs != e->mlp_out) e->drop_exec();  // the captured batch holds the output count
     BW_HIP_CHECK(hipMemcpy(e->mlp, weights, sizeof(float) * mlp_len(outputs), hipMemcpyHostToDevice));
     e->mlp_out = outputs;
     return BW_HIP_OK;
@@ -1177,7 +1220,13 @@ extern "C" int bw_bfs_engine_set_model(bw_bfs_engine *e, const float *weights, i
 extern "C" int bw_bfs_engine_set_schedule(bw_bfs_engine *e, const int32_t *schedule, int32_t len)
 {
     if (!e || !schedule || len < 1) return fail(BW_HIP_ERR_ARG, "invalid schedule");
-    for (int32_t i = 0; i < len; i++) BW_TRY(check_variant(e, schedule[i]));
+    uint32_t kinds = 0;
+    for (int32_t i = 0; i < len; i++) {
+        BW_TRY(check_variant(e, schedule[i]));
+        kinds |= schedule[i] < kRef ? 1u : 2u;
+    }
+    if (kinds != e->schedule_kinds) e->drop_exec();  // the captured batch launches per the kinds
+    e->schedule_kinds = kinds;
     if (len > e->schedule_cap) {
         e->drop_exec();  // the captured batch holds the old schedule pointer
         if (e->schedule) (void)hipFree(e->schedule);
@@ -1366,12 +1415,7 @@ extern "C" int bw_bfs_engine_step_forced(bw_bfs_engine *e, int32_t variant, floa
     BW_HIP_CHECK(hipEventCreate(&ev1));
     int st = BW_HIP_OK;
     if (hipEventRecord(ev0, s) != hipSuccess) st = fail(BW_HIP_ERR_RUNTIME, "event record failed");
-    if (st == BW_HIP_OK) {
-        hipLaunchKernelGGL(bfs_engine_step, dim3(e->grid), dim3(kBlock), sizeof(uint32_t) * e->b.lds_queue, s,
-                           e->g, e->b);
-        hipError_t le = hipGetLastError();
-        if (le != hipSuccess) st = hip_fail(le, "bfs_engine_step", __FILE__, __LINE__);
-    }
+    if (st == BW_HIP_OK) st = launch_step(e, variant >= kRef, s);
     if (st == BW_HIP_OK && hipEventRecord(ev1, s) != hipSuccess) st = fail(BW_HIP_ERR_RUNTIME, "event record failed");
     if (st == BW_HIP_OK) {
         hipLaunchKernelGGL(bfs_engine_epilogue, dim3(1), dim3(kBlock), 0, s, e->g, e->b,

@@ -149,6 +149,29 @@ def pagerank(scale: int, iterations: int, reorder: bool, hot: int = 0, grid: int
     print(f"pagerank: V={g.vertex_count} E={g.edge_count} {iterations} iterations {ms:.3f} ms", flush=True)
 
 
+def pagerank_variants(scale: int, iterations: int, variants) -> None:
+    """The reference's PageRank variants one after another on the directed
+    R-MAT (scale, edge factor 16, seed 2), uniform dangling mass: one
+    warm-up run each, then one timed run (wall clock around the call, the
+    download of the ranks included) for the kernel trace to attribute."""
+    import time
+
+    from . import device, graph, pagerank as pr
+    device.require_gpu()
+    device.set_device(0)
+    g = graph.rmat(scale, 16, seed=2)
+    dg = device.DeviceGraph(g, reps=device.REPRESENTATIONS)
+    for v in variants:
+        pr.run(dg, v, iterations, "uniform")
+        device.synchronize()
+        t0 = time.perf_counter()
+        pr.run(dg, v, iterations, "uniform")
+        dt = time.perf_counter() - t0
+        print(f"pagerank {v}: V={g.vertex_count} E={g.edge_count} {iterations} iterations "
+              f"{dt * 1e3:.3f} ms wall", flush=True)
+    dg.free()
+
+
 def msbfs(scale: int, sources: int, wave_degree: int, topdown_divisor: int, hub_degree: int) -> None:
     from . import bfs_native, device, graph
     device.require_gpu()
@@ -166,18 +189,18 @@ def msbfs(scale: int, sources: int, wave_degree: int, topdown_divisor: int, hub_
     dg.free()
 
 
-def bfs(scale: int, sources: int, policy: str) -> None:
+def bfs(scale: int, sources: int, policy: str, edge_lists: bool = True) -> None:
     from . import bfs_native, device, graph, switch
     device.require_gpu()
     device.set_device(0)
     g = graph.rmat(scale, 16, seed=5, undirected=True)
     dg = device.DeviceGraph(g)
-    eng = bfs_native.BFSEngine(dg)
+    eng = bfs_native.BFSEngine(dg, edge_lists=edge_lists)
     if policy == "learned":
         eng.set_model(switch.Model.load(switch.MODEL_PATH).w)
     def one(s):
-        if policy == "learned":
-            return eng.run(s, "learned", "td", download=False)
+        if policy in ("learned", "beamer"):
+            return eng.run(s, policy, "td", download=False)
         return eng.run(s, "fixed", policy, download=False)
 
     src = graph.random_sources(g, sources, seed=8)
@@ -187,7 +210,7 @@ def bfs(scale: int, sources: int, policy: str) -> None:
         r = one(s)
         total += r.total_ms
         print(f"bfs {policy}: source {s} {r.num_levels} levels {r.total_ms:.3f} ms "
-              f"variants {r.choices.tolist()}", flush=True)
+              f"variants {[bfs_native.VARIANTS[c] for c in r.choices]}", flush=True)
     print(f"bfs {policy}: {len(src)} sources {total:.3f} ms", flush=True)
 
 
@@ -221,7 +244,12 @@ def main(argv=None) -> int:
     b = sub.add_parser("bfs")
     b.add_argument("--scale", type=int, default=25)
     b.add_argument("--sources", type=int, default=4)
-    b.add_argument("--policy", default="learned")
+    b.add_argument("--policy", default="learned", help="learned, beamer or a variant name")
+    b.add_argument("--no-edge-lists", action="store_true", help="engine without the edge-list families")
+    v = sub.add_parser("pagerank-variants")
+    v.add_argument("--scale", type=int, default=22)
+    v.add_argument("--iterations", type=int, default=10)
+    v.add_argument("--variants", default="", help="comma separated (default: all ten)")
     a = ap.parse_args(argv)
     if a.cmd == "copy":
         copy(a.mib, a.repeat)
@@ -231,8 +259,11 @@ def main(argv=None) -> int:
         msbfs(a.scale, a.sources, a.wave_degree, a.topdown_divisor, a.hub_degree)
     elif a.cmd == "pagerank":
         pagerank(a.scale, a.iterations, a.reorder, a.hot, a.grid, a.tiles, a.sorted, a.batch, a.bins)
+    elif a.cmd == "pagerank-variants":
+        from .pagerank import VARIANTS
+        pagerank_variants(a.scale, a.iterations, a.variants.split(",") if a.variants else VARIANTS)
     else:
-        bfs(a.scale, a.sources, a.policy)
+        bfs(a.scale, a.sources, a.policy, not a.no_edge_lists)
     return 0
 
 
