@@ -418,25 +418,48 @@ int vertex_entry(uint64_t V, const void *off, int32_t offset_bytes, const uint32
 
 // ----------------------------------------------------- edge-list builder
 
+// Edge lists from a CSR, edge-parallel: a workgroup takes kListEdges
+// consecutive edges, finds the rows of the first and last by binary search
+// over the offsets, and each thread finds the row of each of its edges by
+// binary search within that range (a few steps over cached offsets; none
+// inside one long row).  Stores are coalesced across the workgroup.
+constexpr uint64_t kListEdges = 4096;
+
+// The row of edge e: the largest r in [lo, hi] with off[r] <= e.
 template<typename Off>
-__global__ void build_edge_lists_kernel(uint64_t V, const Off *__restrict__ off,
-                                        const uint32_t *__restrict__ cols, uint32_t *in,
-                                        uint32_t *out, uint32_t *pairs)
+__device__ __forceinline__ uint64_t row_of(const Off *off, uint64_t lo, uint64_t hi, uint64_t e)
 {
-    // One wave per row: lanes stride over the row's edges.
-    const uint64_t waves = static_cast<uint64_t>(blockDim.x) * gridDim.x / kWave;
-    const uint32_t lane = threadIdx.x & (kWave - 1);
-    for (uint64_t row = (static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x) / kWave; row < V;
-         row += waves) {
-        const uint64_t s = off[row], e = off[row + 1];
-        for (uint64_t j = s + lane; j < e; j += kWave) {
-            const uint32_t c = cols[j];
-            if (in) in[j] = static_cast<uint32_t>(row);
-            if (out) out[j] = c;
-            if (pairs) {
-                pairs[2 * j] = static_cast<uint32_t>(row);
-                pairs[2 * j + 1] = c;
-            }
+    while (lo < hi) {
+        const uint64_t mid = lo + (hi - lo + 1) / 2;
+        if (static_cast<uint64_t>(off[mid]) <= e) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+template<typename Off>
+__global__ __launch_bounds__(256) void build_edge_lists_kernel(uint64_t V, uint64_t E, const Off *__restrict__ off,
+                                                               const uint32_t *__restrict__ cols, uint32_t *in,
+                                                               uint32_t *out, uint2 *pairs)
+{
+    __shared__ uint64_t range[2];
+    for (uint64_t e0 = static_cast<uint64_t>(blockIdx.x) * kListEdges; e0 < E;
+         e0 += static_cast<uint64_t>(gridDim.x) * kListEdges) {
+        const uint64_t e1 = e0 + kListEdges < E ? e0 + kListEdges : E;
+        __syncthreads();  // the previous step has read the range
+        if (threadIdx.x == 0) {
+            const uint64_t r0 = row_of(off, 0, V - 1, e0);
+            range[0] = r0;
+            range[1] = row_of(off, r0, V - 1, e1 - 1);
+        }
+        __syncthreads();
+        const uint64_t r0 = range[0], r1 = range[1];
+        for (uint64_t e = e0 + threadIdx.x; e < e1; e += blockDim.x) {
+            const uint32_t r = static_cast<uint32_t>(row_of(off, r0, r1, e));
+            const uint32_t c = cols[e];
+            if (in) in[e] = r;
+            if (out) out[e] = c;
+            if (pairs) pairs[e] = make_uint2(r, c);
         }
     }
 }
@@ -551,15 +574,16 @@ extern "C" int bw_build_edge_lists(uint64_t V, uint64_t E, const void *off, int3
 {
     if (V == 0 || E == 0) return BW_HIP_OK;
     if (!off || !cols) return fail(BW_HIP_ERR_ARG, "CSR arrays must be device pointers");
-    uint32_t block = 256, grid = 0;
-    BW_TRY(resolve_geometry(nullptr, V * kWave, block, grid));
+    if (reinterpret_cast<uintptr_t>(pairs) % 8) return fail(BW_HIP_ERR_ARG, "pairs must be 8-byte aligned");
+    const uint32_t grid = static_cast<uint32_t>(std::min<uint64_t>((E + kListEdges - 1) / kListEdges, 65536));
     hipStream_t s = static_cast<hipStream_t>(stream);
+    uint2 *p2 = reinterpret_cast<uint2 *>(pairs);  // device allocations are 8-byte aligned
     if (ob == 4)
-        hipLaunchKernelGGL(build_edge_lists_kernel<uint32_t>, dim3(grid), dim3(block), 0, s, V,
-                           static_cast<const uint32_t *>(off), cols, in, out, pairs);
+        hipLaunchKernelGGL(build_edge_lists_kernel<uint32_t>, dim3(grid), dim3(256), 0, s, V, E,
+                           static_cast<const uint32_t *>(off), cols, in, out, p2);
     else if (ob == 8)
-        hipLaunchKernelGGL(build_edge_lists_kernel<uint64_t>, dim3(grid), dim3(block), 0, s, V,
-                           static_cast<const uint64_t *>(off), cols, in, out, pairs);
+        hipLaunchKernelGGL(build_edge_lists_kernel<uint64_t>, dim3(grid), dim3(256), 0, s, V, E,
+                           static_cast<const uint64_t *>(off), cols, in, out, p2);
     else
         return fail(BW_HIP_ERR_ARG, "offset_bytes must be 4 or 8");
     BW_LAUNCH_CHECK();

@@ -74,3 +74,25 @@ def test_fused_pagerank_degenerate(gpu, mode, width):
         assert float(np.max(np.abs(ranks - want))) <= 1e-6, (g.name, mode, width)
         assert np.all(np.isfinite(resid))
         fp.close()
+
+
+@pytest.mark.parametrize("offset_bytes", [4, 8])
+def test_edge_lists_match_the_csr(gpu, offset_bytes):
+    """The edge lists built on the device (bw_build_edge_lists) hold every
+    CSR edge in CSR order: graphs with empty rows, one very long row and a
+    chain, both offset widths, both directions."""
+    graphs = [graph.star(70000), graph.chain(5000), graph.rmat(14, 8, seed=41),
+              graph.from_edges(9000, np.array([8999, 8999, 3], np.uint32), np.array([0, 1, 2], np.uint32))]
+    for g in graphs:
+        dg = device.DeviceGraph(g, offset_bytes=offset_bytes, reps=device.REPRESENTATIONS)
+        try:
+            for rev in (False, True):
+                s, d = g.edges(reverse=rev)
+                ki, ko, kp = ("rel_in", "rel_out", "rsel") if rev else ("el_in", "el_out", "sel")
+                a_in = dg.array(ki).download()[:g.edge_count]
+                a_out = dg.array(ko).download()[:g.edge_count]
+                pairs = dg.array(kp).download()[:2 * g.edge_count]
+                assert np.array_equal(a_in, s) and np.array_equal(a_out, d), (g.name, rev)
+                assert np.array_equal(pairs[0::2], s) and np.array_equal(pairs[1::2], d), (g.name, rev)
+        finally:
+            dg.free()
