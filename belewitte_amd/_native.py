@@ -168,6 +168,8 @@ _HIP_SIGS = {
     "bw_pr_plan_tune": (C.c_int, [vp, C.c_uint32, C.c_uint32, C.c_uint32]),
     "bw_pr_plan_sort_windows": (C.c_int, [vp, C.c_int32, C.c_int32]),
     "bw_pr_plan_bin": (C.c_int, [vp, C.c_int32, C.c_int32, C.c_int32]),
+    "bw_pr_plan_bin_host": (C.c_int, [vp, C.c_int32, C.c_int32, C.c_int32]),
+    "bw_pr_plan_bin_array": (C.c_int, [vp, C.c_int32, vp, C.c_uint64, C.POINTER(C.c_uint64)]),
     "bw_pr_plan_bin_info": (C.c_int, [vp, u32p, u32p, u32p]),
     "bw_pr_fused_acc_len": (C.c_uint64, [C.c_int32]),
     "bw_pr_fused_run": (C.c_int, [_G, vp, C.c_int32, C.c_int32, C.c_int32, vp, vp, vp, f64p, f32p, _L]),

@@ -40,7 +40,7 @@ HIP_FLAGS = [
     f"-I{INC}",
 ]
 
-HIP_SOURCES = ["backend.hip", "bfs.hip", "pagerank.hip", "pagerank_scatter.hip", "bfs_native.hip",
+HIP_SOURCES = ["backend.hip", "bfs.hip", "pagerank.hip", "pagerank_scatter.hip", "pagerank_plan.hip", "bfs_native.hip",
                "pagerank_native.hip", "mlp.hip", "msbfs.hip"]
 
 

@@ -30,6 +30,7 @@
 
 #include "common.hpp"
 #include "pagerank_bins_plan.hpp"
+#include "pagerank_plan.hpp"
 
 #include <algorithm>
 #include <thread>
@@ -927,6 +928,8 @@ struct bw_pr_plan {
     // Binned mode (bw_pr_plan_bin): 2^bin_log2 rows per partition, 0 = off.
     struct Bins {
         uint32_t log2 = 0, chunk_tasks = 0, nchunks = 0, nblocks = 0, pairs = 0, slots = 0, K = 0;
+        uint32_t ntasks = 0, nruns = 0;  // task records; run entries (terminators included)
+        uint64_t ncodes = 0;             // edge codes, padding included
         uint32_t lane_edges = 8;  // edge codes per lane and round of the accumulate kernel
         uint16_t *codes = nullptr;
         uint4 *tasks = nullptr, *chunks = nullptr, *blocks = nullptr;
@@ -1151,24 +1154,34 @@ extern "C" int bw_pr_plan_sort_windows(bw_pr_plan *plan, int32_t tiles_per_windo
     return BW_HIP_OK;
 }
 
-extern "C" int bw_pr_plan_bin(bw_pr_plan *plan, int32_t rows_log2, int32_t chunk_tasks, int32_t lane_edges)
+namespace {
+
+int check_bin_args(const bw_pr_plan *plan, int32_t rows_log2, int32_t chunk_tasks, int32_t lane_edges)
 {
     if (!plan) return fail(BW_HIP_ERR_ARG, "plan is NULL");
-    if (rows_log2 == 0) {
-        plan->binned = false;
-        return BW_HIP_OK;
-    }
-    if (rows_log2 != 13 && rows_log2 != 14) return fail(BW_HIP_ERR_ARG, "rows_log2 must be 0, 13 or 14");
+    if (rows_log2 != 0 && rows_log2 != 13 && rows_log2 != 14)
+        return fail(BW_HIP_ERR_ARG, "rows_log2 must be 0, 13 or 14");
+    if (rows_log2 == 0) return BW_HIP_OK;
     if (chunk_tasks < 1 || chunk_tasks > 4096) return fail(BW_HIP_ERR_ARG, "chunk_tasks must be in [1, 4096]");
     if (lane_edges != 8 && lane_edges != 16) return fail(BW_HIP_ERR_ARG, "lane_edges must be 8 or 16");
-    auto &B = plan->bins;
-    if (B.log2 == static_cast<uint32_t>(rows_log2) && B.chunk_tasks == static_cast<uint32_t>(chunk_tasks)) {
-        B.lane_edges = static_cast<uint32_t>(lane_edges);
-        plan->binned = true;
-        return BW_HIP_OK;
-    }
-    // A new shape is built beside the current one and replaces it only once
-    // it is complete: a failed call leaves the plan as it was.
+    return BW_HIP_OK;
+}
+
+// The per-iteration buffers of a plan with `pairs` pairs and `slots` partials.
+int alloc_bin_buffers(bw_pr_plan::Bins &nb, uint32_t rows_log2, uint32_t pairs, uint32_t slots, uint32_t nblocks)
+{
+    const size_t rows = size_t{1} << rows_log2;
+    // 8 values of padding: the accumulate reads 8-pair windows from a lane's first pair
+    if (hipMalloc(&nb.vals, sizeof(float) * (static_cast<size_t>(pairs) + 8)) != hipSuccess ||
+        hipMalloc(&nb.partials, sizeof(float) * rows * std::max<size_t>(slots, 1)) != hipSuccess ||
+        hipMalloc(&nb.part, sizeof(double2) * std::max<size_t>(nblocks, 1)) != hipSuccess)
+        return BW_HIP_ERR_NOMEM;
+    return BW_HIP_OK;
+}
+
+// Host build (bins::build_plan): the reverse CSR is copied to the host.
+int bin_host(bw_pr_plan *plan, uint32_t rows_log2, uint32_t chunk_tasks, bw_pr_plan::Bins &nb)
+{
     const uint32_t V = plan->V, E = plan->E;
     std::vector<uint32_t> off(static_cast<size_t>(V) + 1), cols(std::max<uint32_t>(E, 1));
     BW_HIP_CHECK(hipMemcpy(off.data(), plan->roff, sizeof(uint32_t) * (static_cast<size_t>(V) + 1),
@@ -1176,8 +1189,7 @@ extern "C" int bw_pr_plan_bin(bw_pr_plan *plan, int32_t rows_log2, int32_t chunk
     if (E) BW_HIP_CHECK(hipMemcpy(cols.data(), plan->rcols, sizeof(uint32_t) * E, hipMemcpyDeviceToHost));
     bins::Plan P;
     std::string err;
-    if (!bins::build_plan(off.data(), cols.data(), V, E, static_cast<uint32_t>(rows_log2),
-                          static_cast<uint32_t>(chunk_tasks), P, err))
+    if (!bins::build_plan(off.data(), cols.data(), V, E, rows_log2, chunk_tasks, P, err))
         return fail(BW_HIP_ERR_ARG, "binned PageRank plan: " + err);
     static_assert(sizeof(bins::Quad) == sizeof(uint4), "Quad and uint4 share a layout");
     bool ok = true;
@@ -1191,7 +1203,6 @@ extern "C" int bw_pr_plan_bin(bw_pr_plan *plan, int32_t rows_log2, int32_t chunk
         if (!vec.empty() && hipMemcpy(dst, vec.data(), sizeof(T) * vec.size(), hipMemcpyHostToDevice) != hipSuccess)
             ok = false;
     };
-    bw_pr_plan::Bins nb;
     put(nb.codes, P.codes);
     put(nb.tasks, P.tasks);
     put(nb.chunks, P.chunks);
@@ -1200,25 +1211,114 @@ extern "C" int bw_pr_plan_bin(bw_pr_plan *plan, int32_t rows_log2, int32_t chunk
     put(nb.run_flat, P.run_flat);
     put(nb.run_bin, P.run_bin);
     put(nb.psrc, P.psrc);
-    const size_t rows = size_t{1} << rows_log2;
-    // 8 values of padding: the accumulate reads 8-pair windows from a lane's first pair
-    if (!ok || hipMalloc(&nb.vals, sizeof(float) * (static_cast<size_t>(P.pairs) + 8)) != hipSuccess ||
-        hipMalloc(&nb.partials, sizeof(float) * rows * std::max<size_t>(P.slots, 1)) != hipSuccess ||
-        hipMalloc(&nb.part, sizeof(double2) * std::max<size_t>(P.blocks.size(), 1)) != hipSuccess) {
-        nb.release();
+    if (!ok || alloc_bin_buffers(nb, rows_log2, P.pairs, P.slots, static_cast<uint32_t>(P.blocks.size())) != BW_HIP_OK)
         return fail(BW_HIP_ERR_NOMEM, "device allocation or upload of the binned PageRank plan failed");
-    }
-    nb.log2 = static_cast<uint32_t>(rows_log2);
-    nb.chunk_tasks = static_cast<uint32_t>(chunk_tasks);
-    nb.lane_edges = static_cast<uint32_t>(lane_edges);
     nb.nchunks = static_cast<uint32_t>(P.chunks.size());
     nb.nblocks = static_cast<uint32_t>(P.blocks.size());
     nb.pairs = P.pairs;
     nb.slots = P.slots;
     nb.K = P.K;
+    nb.ntasks = static_cast<uint32_t>(P.tasks.size());
+    nb.nruns = static_cast<uint32_t>(P.run_flat.size());
+    nb.ncodes = P.codes.size();
+    return BW_HIP_OK;
+}
+
+// Device build (bins::build_plan_device): same arrays, no copy to the host.
+int bin_device(bw_pr_plan *plan, uint32_t rows_log2, uint32_t chunk_tasks, bw_pr_plan::Bins &nb)
+{
+    bins::DeviceBins d;
+    std::string err;
+    const int st = bins::build_plan_device(static_cast<const uint32_t *>(plan->roff), plan->rcols, plan->V, plan->E,
+                                           rows_log2, chunk_tasks, nullptr, d, err);
+    if (st != BW_HIP_OK) return fail(st, "binned PageRank plan: " + err);
+    nb.codes = d.codes;
+    nb.tasks = d.tasks;
+    nb.chunks = d.chunks;
+    nb.blocks = d.blocks;
+    nb.chunk_base = d.chunk_base;
+    nb.run_flat = d.run_flat;
+    nb.run_bin = d.run_bin;
+    nb.psrc = d.psrc;
+    if (alloc_bin_buffers(nb, rows_log2, d.pairs, d.slots, d.nblocks) != BW_HIP_OK)
+        return fail(BW_HIP_ERR_NOMEM, "device allocation of the binned PageRank buffers failed");
+    nb.nchunks = d.nchunks;
+    nb.nblocks = d.nblocks;
+    nb.pairs = d.pairs;
+    nb.slots = d.slots;
+    nb.K = d.K;
+    nb.ntasks = d.ntasks;
+    nb.nruns = d.nruns;
+    nb.ncodes = d.ncodes;
+    return BW_HIP_OK;
+}
+
+int bin_impl(bw_pr_plan *plan, int32_t rows_log2, int32_t chunk_tasks, int32_t lane_edges, bool on_host)
+{
+    BW_TRY(check_bin_args(plan, rows_log2, chunk_tasks, lane_edges));
+    if (rows_log2 == 0) {
+        plan->binned = false;
+        return BW_HIP_OK;
+    }
+    auto &B = plan->bins;
+    if (!on_host && B.log2 == static_cast<uint32_t>(rows_log2) && B.chunk_tasks == static_cast<uint32_t>(chunk_tasks)) {
+        B.lane_edges = static_cast<uint32_t>(lane_edges);
+        plan->binned = true;
+        return BW_HIP_OK;
+    }
+    // A new shape is built beside the current one and replaces it only once
+    // it is complete: a failed call leaves the plan as it was.
+    bw_pr_plan::Bins nb;
+    const int st = on_host ? bin_host(plan, static_cast<uint32_t>(rows_log2), static_cast<uint32_t>(chunk_tasks), nb)
+                           : bin_device(plan, static_cast<uint32_t>(rows_log2), static_cast<uint32_t>(chunk_tasks), nb);
+    if (st != BW_HIP_OK) {
+        nb.release();
+        return st;
+    }
+    nb.log2 = static_cast<uint32_t>(rows_log2);
+    nb.chunk_tasks = static_cast<uint32_t>(chunk_tasks);
+    nb.lane_edges = static_cast<uint32_t>(lane_edges);
     B.release();
     B = nb;
     plan->binned = true;
+    return BW_HIP_OK;
+}
+
+}  // namespace
+
+extern "C" int bw_pr_plan_bin(bw_pr_plan *plan, int32_t rows_log2, int32_t chunk_tasks, int32_t lane_edges)
+{
+    return bin_impl(plan, rows_log2, chunk_tasks, lane_edges, false);
+}
+
+extern "C" int bw_pr_plan_bin_host(bw_pr_plan *plan, int32_t rows_log2, int32_t chunk_tasks, int32_t lane_edges)
+{
+    return bin_impl(plan, rows_log2, chunk_tasks, lane_edges, true);
+}
+
+extern "C" int bw_pr_plan_bin_array(const bw_pr_plan *plan, int32_t which, void *host_dst, uint64_t capacity_bytes,
+                                    uint64_t *bytes)
+{
+    if (!plan || !bytes) return fail(BW_HIP_ERR_ARG, "invalid argument");
+    if (!plan->binned) return fail(BW_HIP_ERR_ARG, "the plan is not binned");
+    const auto &B = plan->bins;
+    const void *src = nullptr;
+    uint64_t n = 0;
+    switch (which) {
+    case BW_PR_BIN_CODES: src = B.codes; n = sizeof(uint16_t) * B.ncodes; break;
+    case BW_PR_BIN_TASKS: src = B.tasks; n = sizeof(uint4) * B.ntasks; break;
+    case BW_PR_BIN_CHUNKS: src = B.chunks; n = sizeof(uint4) * B.nchunks; break;
+    case BW_PR_BIN_CHUNK_BASE: src = B.chunk_base; n = sizeof(uint32_t) * (static_cast<uint64_t>(B.K) + 1); break;
+    case BW_PR_BIN_BLOCKS: src = B.blocks; n = sizeof(uint4) * B.nblocks; break;
+    case BW_PR_BIN_RUN_FLAT: src = B.run_flat; n = sizeof(uint32_t) * B.nruns; break;
+    case BW_PR_BIN_RUN_BIN: src = B.run_bin; n = sizeof(uint32_t) * B.nruns; break;
+    case BW_PR_BIN_PSRC: src = B.psrc; n = sizeof(uint16_t) * B.pairs; break;
+    default: return fail(BW_HIP_ERR_ARG, "unknown plan array " + std::to_string(which));
+    }
+    *bytes = n;
+    if (!host_dst) return BW_HIP_OK;
+    if (capacity_bytes < n) return fail(BW_HIP_ERR_ARG, "destination too small for the plan array");
+    if (n) BW_HIP_CHECK(hipMemcpy(host_dst, src, n, hipMemcpyDeviceToHost));
     return BW_HIP_OK;
 }
 

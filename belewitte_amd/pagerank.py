@@ -145,16 +145,34 @@ class FusedPageRank:
         N.check_hip(lib.bw_pr_plan_sort_windows(self._plan, int(sorted_windows), int(batch)))
         self.tuning = (int(hot), int(rows_grid), int(tiles), int(sorted_windows), int(batch))
 
-    def bin(self, rows_log2: int = 14, chunk_tasks: int = 128, lane_edges: int = 8) -> None:
+    def bin(self, rows_log2: int = 14, chunk_tasks: int = 128, lane_edges: int = 8, on_host: bool = False) -> None:
         """Switches to the binned iteration (propagation blocking): rows cut
         into partitions of 2**rows_log2 (13, 14) accumulated in LDS in FP64,
         one contribution per distinct (source, partition) pair written and
         read per iteration, partitions' edges cut into chunks of
         chunk_tasks x 2048 edges, lane_edges (8, 16) edges per lane and
         round.  rows_log2=0 returns to the pull kernels.  The plan is built
-        on the host on the first call for a partition and chunk size."""
-        N.check_hip(N.hip_lib().bw_pr_plan_bin(self._plan, int(rows_log2), int(chunk_tasks), int(lane_edges)))
+        on the device on the first call for a partition and chunk size;
+        on_host=True builds it on the host instead (always anew)."""
+        fn = N.hip_lib().bw_pr_plan_bin_host if on_host else N.hip_lib().bw_pr_plan_bin
+        N.check_hip(fn(self._plan, int(rows_log2), int(chunk_tasks), int(lane_edges)))
         self.binning = (int(rows_log2), int(chunk_tasks), int(lane_edges)) if rows_log2 else None
+
+    BIN_ARRAYS = {"codes": (0, np.uint16), "tasks": (1, np.uint32), "chunks": (2, np.uint32),
+                  "chunk_base": (3, np.uint32), "blocks": (4, np.uint32), "run_flat": (5, np.uint32),
+                  "run_bin": (6, np.uint32), "psrc": (7, np.uint16)}
+
+    def bin_array(self, name: str) -> np.ndarray:
+        """A copy of one array of the binned plan (BIN_ARRAYS; uint4 records
+        come as 4 uint32 each)."""
+        which, dt = self.BIN_ARRAYS[name]
+        n = C.c_uint64()
+        lib = N.hip_lib()
+        N.check_hip(lib.bw_pr_plan_bin_array(self._plan, which, None, 0, C.byref(n)))
+        out = np.empty(int(n.value) // np.dtype(dt).itemsize, dtype=dt)
+        N.check_hip(lib.bw_pr_plan_bin_array(self._plan, which, out.ctypes.data_as(C.c_void_p), out.nbytes,
+                                             C.byref(n)))
+        return out
 
     def bin_info(self) -> dict:
         a, b, c = C.c_uint32(), C.c_uint32(), C.c_uint32()

@@ -470,8 +470,9 @@ int bw_pr_plan_tune(bw_pr_plan *plan, uint32_t hot_vertices,
 int bw_pr_plan_sort_windows(bw_pr_plan *plan, int32_t tiles_per_window,
                             int32_t gathers_per_round);
 /* rows_log2 13 or 14: switches the plan to the binned iteration
- * (propagation blocking), built on the host on the first call for a
- * partition size and chunk size from the graph's reverse CSR given to
+ * (propagation blocking), built on the device (stable radix sorts and scans
+ * over the reverse CSR, no copy of the graph to the host) on the first call
+ * for a partition size and chunk size from the graph's reverse CSR given to
  * bw_pr_plan_create (which must still be live).  Rows are cut into
  * partitions of 2^rows_log2; each distinct (source, partition) pair holds
  * one contribution per iteration, written once by the source and read once
@@ -487,6 +488,24 @@ int bw_pr_plan_sort_windows(bw_pr_plan *plan, int32_t tiles_per_window,
  * it had before the call. */
 int bw_pr_plan_bin(bw_pr_plan *plan, int32_t rows_log2, int32_t chunk_tasks,
                    int32_t lane_edges);
+/* The same plan built on the host from a copy of the reverse CSR (the
+ * sequential builder the device build restates); always rebuilds.  No
+ * counterpart in the reference: for checking the device build and for
+ * callers that want the host's memory instead of the device's scratch. */
+int bw_pr_plan_bin_host(bw_pr_plan *plan, int32_t rows_log2,
+                        int32_t chunk_tasks, int32_t lane_edges);
+/* Copies one array of the current binned plan to host memory: its size in
+ * bytes goes to *bytes; host_dst may be NULL to ask for the size only. */
+#define BW_PR_BIN_CODES 0      /* uint16 per edge, partition-major, padded */
+#define BW_PR_BIN_TASKS 1      /* uint4 (edge begin, edge end, pair base, partition) */
+#define BW_PR_BIN_CHUNKS 2     /* uint4 (task begin, task end, partition, slot) */
+#define BW_PR_BIN_CHUNK_BASE 3 /* uint32[K + 1] */
+#define BW_PR_BIN_BLOCKS 4     /* uint4 (first vertex, end vertex, run begin, run count) */
+#define BW_PR_BIN_RUN_FLAT 5   /* uint32 per run entry */
+#define BW_PR_BIN_RUN_BIN 6    /* uint32 per run entry */
+#define BW_PR_BIN_PSRC 7       /* uint16 per pair, scatter order */
+int bw_pr_plan_bin_array(const bw_pr_plan *plan, int32_t which, void *host_dst,
+                         uint64_t capacity_bytes, uint64_t *bytes);
 /* Pairs, chunks and scatter blocks of the binned plan (0 when off). */
 int bw_pr_plan_bin_info(const bw_pr_plan *plan, uint32_t *pairs,
                         uint32_t *chunks, uint32_t *blocks);

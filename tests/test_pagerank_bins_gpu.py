@@ -159,3 +159,58 @@ def test_binned_agrees_with_pull_at_scale_20(gpu, shape):
     assert abs(float(np.sum(ranks, dtype=np.float64)) - 1.0) <= 1e-5
     assert np.allclose(resid, rp, rtol=1e-3, atol=1e-6)
     fp.close()
+
+
+def _plan_graphs():
+    yield graph.rmat(16, 16, seed=51)
+    yield graph.rmat(17, 8, seed=52, undirected=True)
+    g, _ = graph.rmat(16, 32, seed=53).reorder_by_degree()
+    yield g
+    yield _hub_graph()
+    yield graph.erdos_renyi(30000, 200000, seed=54, undirected=False)
+    yield graph.grid2d(300, 200, rewire=0.05, seed=55)
+    yield graph.star(70000)
+    yield graph.chain(40000)
+    yield from degenerate_graphs()
+
+
+@pytest.mark.parametrize("shape", [(13, 1, 8), (13, 64, 8), (14, 3, 16), (14, 128, 8)])
+def test_device_plan_equals_host_plan(gpu, shape):
+    """The plan built on the device (sorts, scans and the greedy block cut
+    on the GPU) is the host builder's, array for array and bit for bit."""
+    for g in _plan_graphs():
+        dg = device.DeviceGraph(g)
+        fp = pagerank.FusedPageRank(dg, max_iterations=2)
+        try:
+            print(f"plan {g.name} {shape}", flush=True)
+            fp.bin(*shape)
+            dev = {k: fp.bin_array(k) for k in pagerank.FusedPageRank.BIN_ARRAYS}
+            info_dev = fp.bin_info()
+            fp.bin(*shape, on_host=True)
+            host = {k: fp.bin_array(k) for k in pagerank.FusedPageRank.BIN_ARRAYS}
+            assert fp.bin_info() == info_dev, g.name
+            for k in dev:
+                assert dev[k].shape == host[k].shape, (g.name, shape, k, dev[k].shape, host[k].shape)
+                assert np.array_equal(dev[k], host[k]), (g.name, shape, k)
+        finally:
+            fp.close()
+            dg.free()
+
+
+def test_device_plan_ranks_at_scale_20(gpu):
+    """The device-built plan at R-MAT scale 20 (degree-ordered) against the
+    oracle, both modes."""
+    g0 = graph.rmat(20, 16, seed=56)
+    g, perm = g0.reorder_by_degree()
+    csr = O.CSR.from_graph(g)
+    dg = device.DeviceGraph(g)
+    fp = pagerank.FusedPageRank(dg, max_iterations=30)
+    try:
+        for shape in ((14, 128, 8), (13, 64, 8)):
+            fp.bin(*shape)
+            for mode, quirk, iters in (("uniform", False, 30), ("drop", True, 20)):
+                ranks, resid, _ = fp.run(iters, mode, nodiv_quirk=quirk)
+                _check(g, csr, ranks, resid, mode, quirk, iters)
+    finally:
+        fp.close()
+        dg.free()
