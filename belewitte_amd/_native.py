@@ -130,6 +130,7 @@ _HIP_SIGS = {
     "bw_bfs_vertex_push_warp": (C.c_int, [C.c_uint64, vp, C.c_int32, vp, vp, C.c_int32, vp, C.c_int32, _L]),
     "bw_bfs_vertex_pull_warp": (C.c_int, [C.c_uint64, vp, C.c_int32, vp, vp, C.c_int32, vp, C.c_int32, _L]),
     "bw_bfs_level": (C.c_int, [_G, C.c_int32, C.c_int32, vp, C.c_int32, vp, _L]),
+    "bw_graph_relabel_by_degree": (C.c_int, [C.c_uint64, C.c_uint64, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp]),
     "bw_build_edge_lists": (C.c_int, [C.c_uint64, C.c_uint64, vp, C.c_int32, vp, vp, vp, vp, vp]),
     "bw_bfs_run": (C.c_int, [_G, i32p, C.c_int32, C.c_int32, C.c_uint32, vp, u32p, f32p, C.c_int64, i64p, _L]),
     "bw_bfs_run_static": (C.c_int, [_G, i32p, C.c_int32, C.c_int32, C.c_uint32, vp, u32p, C.c_int64, i64p, _L]),

@@ -760,5 +760,205 @@ int build_plan_device(const uint32_t *roff, const uint32_t *rcols, uint32_t V, u
     return BW_HIP_OK;
 }
 
+// ------------------------------------------------ relabel by out-degree
+
+namespace {
+
+// key = ~out-degree, so an ascending stable sort puts larger degrees first
+// and keeps equal degrees in id order.
+__global__ __launch_bounds__(kThreads) void relabel_keys_kernel(uint32_t V, const uint32_t *off, uint32_t *key,
+                                                                uint32_t *iota)
+{
+    for (uint64_t v = static_cast<uint64_t>(blockIdx.x) * kThreads + threadIdx.x; v < V;
+         v += static_cast<uint64_t>(gridDim.x) * kThreads) {
+        key[v] = ~(off[v + 1] - off[v]);
+        iota[v] = static_cast<uint32_t>(v);
+    }
+}
+
+__global__ __launch_bounds__(kThreads) void relabel_perm_kernel(uint32_t V, const uint32_t *order, uint32_t *perm)
+{
+    for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * kThreads + threadIdx.x; i < V;
+         i += static_cast<uint64_t>(gridDim.x) * kThreads)
+        perm[order[i]] = static_cast<uint32_t>(i);
+}
+
+// ndeg[i] = degree of old row order[i]; ndeg[V] = 0 (for the scan).
+__global__ __launch_bounds__(kThreads) void relabel_degrees_kernel(uint32_t V, const uint32_t *off,
+                                                                   const uint32_t *order, uint32_t *ndeg)
+{
+    for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * kThreads + threadIdx.x; i <= V;
+         i += static_cast<uint64_t>(gridDim.x) * kThreads) {
+        uint32_t d = 0;
+        if (i < V) {
+            const uint32_t o = order[i];
+            d = off[o + 1] - off[o];
+        }
+        ndeg[i] = d;
+    }
+}
+
+// key[j] = (new row << 32) | new column, for new edge j (rows found by
+// binary search over the new offsets within each workgroup's range).
+__global__ __launch_bounds__(kThreads) void relabel_edges_kernel(uint32_t V, uint32_t E, const uint32_t *off,
+                                                                 const uint32_t *cols, const uint32_t *order,
+                                                                 const uint32_t *perm, const uint32_t *noff,
+                                                                 uint64_t *key)
+{
+    __shared__ uint32_t range[2];
+    for (uint64_t e0 = static_cast<uint64_t>(blockIdx.x) * kEdgesPerStep; e0 < E;
+         e0 += static_cast<uint64_t>(gridDim.x) * kEdgesPerStep) {
+        const uint32_t a = static_cast<uint32_t>(e0);
+        const uint32_t b = static_cast<uint32_t>(std::min<uint64_t>(e0 + kEdgesPerStep, E));
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            const uint32_t r0 = row_of(noff, 0, V - 1, a);
+            range[0] = r0;
+            range[1] = row_of(noff, r0, V - 1, b - 1);
+        }
+        __syncthreads();
+        const uint32_t r0 = range[0], r1 = range[1];
+        for (uint32_t j = a + threadIdx.x; j < b; j += kThreads) {
+            const uint32_t i = row_of(noff, r0, r1, j);
+            const uint32_t old = off[order[i]] + (j - noff[i]);
+            key[j] = (static_cast<uint64_t>(i) << 32) | perm[cols[old]];
+        }
+    }
+}
+
+__global__ __launch_bounds__(kThreads) void relabel_cols_kernel(uint32_t E, const uint64_t *key, uint32_t *ncols)
+{
+    for (uint64_t j = static_cast<uint64_t>(blockIdx.x) * kThreads + threadIdx.x; j < E;
+         j += static_cast<uint64_t>(gridDim.x) * kThreads)
+        ncols[j] = static_cast<uint32_t>(key[j]);
+}
+
+// One stored direction: rows permuted by `order`, columns mapped by `perm`
+// and sorted within each row.
+int relabel_direction(Scratch &sc, uint32_t V, uint32_t E, const uint32_t *off, const uint32_t *cols,
+                      const uint32_t *order, const uint32_t *perm, uint32_t *noff, uint32_t *ncols, hipStream_t s,
+                      std::string &err)
+{
+    uint32_t *ndeg = sc.get<uint32_t>(static_cast<uint64_t>(V) + 1);
+    uint64_t *key = sc.get<uint64_t>(E);
+    uint64_t *skey = sc.get<uint64_t>(E);
+    if (!ndeg || !key || !skey) {
+        err = "device allocation of the relabel failed";
+        return BW_HIP_ERR_NOMEM;
+    }
+    hipLaunchKernelGGL(relabel_degrees_kernel, dim3(grid_for(static_cast<uint64_t>(V) + 1)), dim3(kThreads), 0, s, V,
+                       off, order, ndeg);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = exclusive_scan(sc, ndeg, noff, static_cast<uint64_t>(V) + 1, s);
+    if (e == hipSuccess && E) {
+        hipLaunchKernelGGL(relabel_edges_kernel, dim3(grid_for(E, kEdgesPerStep)), dim3(kThreads), 0, s, V, E, off,
+                           cols, order, perm, noff, key);
+        e = hipGetLastError();
+        size_t bytes = 0;
+        const uint32_t bits = 32 + bits_for(V - 1);
+        if (e == hipSuccess) e = rocprim::radix_sort_keys(nullptr, bytes, key, skey, E, 0, bits, s);
+        void *tmp = e == hipSuccess ? sc.get<uint8_t>(bytes) : nullptr;
+        if (e == hipSuccess && !tmp) e = hipErrorOutOfMemory;
+        if (e == hipSuccess) e = rocprim::radix_sort_keys(tmp, bytes, key, skey, E, 0, bits, s);
+        if (e == hipSuccess) {
+            hipLaunchKernelGGL(relabel_cols_kernel, dim3(grid_for(E)), dim3(kThreads), 0, s, E, skey, ncols);
+            e = hipGetLastError();
+        }
+    }
+    if (e != hipSuccess) {
+        err = std::string("relabel: ") + hipGetErrorString(e);
+        return e == hipErrorOutOfMemory ? BW_HIP_ERR_NOMEM : BW_HIP_ERR_RUNTIME;
+    }
+    return BW_HIP_OK;
+}
+
+}  // namespace
+
+namespace {
+
+__global__ __launch_bounds__(kThreads) void check_cols_kernel(uint32_t V, uint32_t E, const uint32_t *cols,
+                                                              uint32_t *bad)
+{
+    for (uint64_t j = static_cast<uint64_t>(blockIdx.x) * kThreads + threadIdx.x; j < E;
+         j += static_cast<uint64_t>(gridDim.x) * kThreads)
+        if (cols[j] >= V) *bad = 1u;
+}
+
+}  // namespace
+
+int relabel_by_degree_device(uint32_t V, uint32_t E, const uint32_t *off, const uint32_t *cols, const uint32_t *roff,
+                             const uint32_t *rcols, uint32_t *noff, uint32_t *ncols, uint32_t *nroff, uint32_t *nrcols,
+                             uint32_t *perm_out, hipStream_t s, std::string &err)
+{
+    Scratch sc;
+    if (E) {  // every column must name a vertex before it indexes the permutation
+        uint32_t *bad = sc.get<uint32_t>(1), hbad = 0;
+        if (!bad || hipMemsetAsync(bad, 0, sizeof(uint32_t), s) != hipSuccess) {
+            err = "device allocation of the relabel failed";
+            return BW_HIP_ERR_NOMEM;
+        }
+        hipLaunchKernelGGL(check_cols_kernel, dim3(grid_for(E)), dim3(kThreads), 0, s, V, E, cols, bad);
+        if (roff) hipLaunchKernelGGL(check_cols_kernel, dim3(grid_for(E)), dim3(kThreads), 0, s, V, E, rcols, bad);
+        if (hipGetLastError() != hipSuccess ||
+            hipMemcpyAsync(&hbad, bad, sizeof hbad, hipMemcpyDeviceToHost, s) != hipSuccess ||
+            hipStreamSynchronize(s) != hipSuccess) {
+            err = "relabel: column check failed on the device";
+            return BW_HIP_ERR_RUNTIME;
+        }
+        if (hbad) {
+            err = "column id out of range";
+            return BW_HIP_ERR_ARG;
+        }
+    }
+    uint32_t *key = sc.get<uint32_t>(V), *skey = sc.get<uint32_t>(V), *iota = sc.get<uint32_t>(V);
+    uint32_t *order = sc.get<uint32_t>(V);
+    uint32_t *perm = perm_out ? perm_out : sc.get<uint32_t>(V);
+    if (!key || !skey || !iota || !order || !perm) {
+        err = "device allocation of the relabel failed";
+        return BW_HIP_ERR_NOMEM;
+    }
+    hipLaunchKernelGGL(relabel_keys_kernel, dim3(grid_for(V)), dim3(kThreads), 0, s, V, off, key, iota);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = sort_pairs(sc, static_cast<const uint32_t *>(key), skey, static_cast<const uint32_t *>(iota),
+                                        order, V, 32, s);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(relabel_perm_kernel, dim3(grid_for(V)), dim3(kThreads), 0, s, V, order, perm);
+        e = hipGetLastError();
+    }
+    if (e != hipSuccess) {
+        err = std::string("relabel: ") + hipGetErrorString(e);
+        return e == hipErrorOutOfMemory ? BW_HIP_ERR_NOMEM : BW_HIP_ERR_RUNTIME;
+    }
+    BW_TRY(relabel_direction(sc, V, E, off, cols, order, perm, noff, ncols, s, err));
+    if (roff) BW_TRY(relabel_direction(sc, V, E, roff, rcols, order, perm, nroff, nrcols, s, err));
+    if (hipStreamSynchronize(s) != hipSuccess) {
+        err = "relabel failed on the device";
+        return BW_HIP_ERR_RUNTIME;
+    }
+    return BW_HIP_OK;
+}
+
 }  // namespace bins
 }  // namespace bw
+
+using namespace bw;
+
+extern "C" int bw_graph_relabel_by_degree(uint64_t V, uint64_t E, const uint32_t *offsets, const uint32_t *cols,
+                                          const uint32_t *rev_offsets, const uint32_t *rev_cols, uint32_t *new_offsets,
+                                          uint32_t *new_cols, uint32_t *new_rev_offsets, uint32_t *new_rev_cols,
+                                          uint32_t *perm, void *stream)
+{
+    if (V == 0 || V >= 0x7FFFFFFFull || E >= 0xFFFFFFFFull)
+        return fail(BW_HIP_ERR_ARG, "relabel needs 0 < V < 2^31 and E < 2^32 - 1");
+    if (!offsets || !new_offsets || (E && (!cols || !new_cols)))
+        return fail(BW_HIP_ERR_ARG, "bw_graph_relabel_by_degree: missing forward arrays");
+    if ((rev_offsets == nullptr) != (new_rev_offsets == nullptr) ||
+        (rev_offsets && E && (!rev_cols || !new_rev_cols)))
+        return fail(BW_HIP_ERR_ARG, "bw_graph_relabel_by_degree: reverse arrays must be given in and out together");
+    std::string err;
+    const int st = bins::relabel_by_degree_device(static_cast<uint32_t>(V), static_cast<uint32_t>(E), offsets, cols,
+                                                  rev_offsets, rev_cols, new_offsets, new_cols, new_rev_offsets,
+                                                  new_rev_cols, perm, static_cast<hipStream_t>(stream), err);
+    if (st != BW_HIP_OK) return fail(st, err);
+    return BW_HIP_OK;
+}

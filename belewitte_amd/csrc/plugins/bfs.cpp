@@ -240,8 +240,9 @@ static auto insertVariant()
 namespace {
 
 // Reads the "weights" array of a switch model file (the JSON written by
-// belewitte_amd.switch.Model.save).
-std::vector<float> readModel(const std::string &path, int32_t outputs)
+// belewitte_amd.switch.Model.save); *outputs receives its output count:
+// one per native variant or one per engine variant.
+std::vector<float> readModel(const std::string &path, int32_t *outputs)
 {
     std::ifstream in(path);
     checkError(in.good(), "Cannot open model file '", path, "'");
@@ -261,8 +262,12 @@ std::vector<float> readModel(const std::string &path, int32_t outputs)
         w.push_back(x);
         p = end;
     }
-    checkError(static_cast<int32_t>(w.size()) == bw_bfs_engine_mlp_len(outputs), "'", path, "' holds ",
-               w.size(), " weights, expected ", bw_bfs_engine_mlp_len(outputs));
+    const int32_t n = static_cast<int32_t>(w.size());
+    if (n == bw_bfs_engine_mlp_len(BW_BFS_ENGINE_VARIANTS)) *outputs = BW_BFS_ENGINE_VARIANTS;
+    else if (n == bw_bfs_engine_mlp_len(BW_BFS_NATIVE_VARIANTS)) *outputs = BW_BFS_NATIVE_VARIANTS;
+    else
+        reportError("'", path, "' holds ", w.size(), " weights, expected ",
+                    bw_bfs_engine_mlp_len(BW_BFS_ENGINE_VARIANTS), " or ", bw_bfs_engine_mlp_len(BW_BFS_NATIVE_VARIANTS));
     return w;
 }
 
@@ -278,13 +283,22 @@ std::string defaultModelPath()
     return dir + "/../../data/bfs_switch_model.json";
 }
 
+// Engine variant ids: the native ones, then the reference's families at
+// BW_BFS_ENGINE_REFERENCE + their bfs/ kernel id.
 int32_t nativeVariant(const std::string &name)
 {
-    if (name == "td") return BW_BFS_NATIVE_TD;
-    if (name == "bu") return BW_BFS_NATIVE_BU;
-    if (name == "bu-wave") return BW_BFS_NATIVE_BU_WAVE;
-    reportError("Unknown native BFS variant '", name, "' (td, bu, bu-wave)");
+    static const char *const names[BW_BFS_ENGINE_VARIANTS] = {
+        "td", "bu", "bu-wave", "edge-list", "rev-edge-list", "struct-edge-list", "rev-struct-edge-list",
+        "vertex-push", "vertex-pull", "vertex-push-warp", "vertex-pull-warp"};
+    for (int32_t i = 0; i < BW_BFS_ENGINE_VARIANTS; i++)
+        if (name == names[i]) return i;
+    reportError("Unknown BFS engine variant '", name,
+                "' (td, bu, bu-wave, or a reference family: edge-list, rev-edge-list, struct-edge-list, "
+                "rev-struct-edge-list, vertex-push, vertex-pull, vertex-push-warp, vertex-pull-warp)");
 }
+
+bool isEdgeListFamily(int32_t v)
+{ return v >= BW_BFS_ENGINE_REFERENCE && v <= BW_BFS_ENGINE_REFERENCE + BW_BFS_REV_STRUCT_EDGE_LIST; }
 
 }  // namespace
 
@@ -308,7 +322,8 @@ struct NativeBFS : public ImplementationBase {
     explicit NativeBFS(int32_t p) : policy(p)
     {
         options.add('r', "root", "NUM", root, "Starting vertex for BFS.");
-        options.add("variant", "NAME", variant, "Fixed variant (td, bu, bu-wave), also the learned fallback.");
+        options.add("variant", "NAME", variant,
+                    "Fixed variant (td, bu, bu-wave or a reference family), also the learned fallback.");
         options.add("model", "FILE", modelPath, "Switch model (default: the one shipped with the library).");
         options.add("schedule", "LIST", schedule, "Comma-separated per-level variants, repeated.");
         options.add("levels-per-sync", "NUM", levelsPerSync, "Levels launched between done-flag reads.");
@@ -348,6 +363,28 @@ struct NativeBFS : public ImplementationBase {
         }
     }
 
+    void buildEdgeLists()
+    {
+        const uint64_t V = dev.vertex_count, E = dev.edge_count;
+        auto alloc = [&](size_t bytes) {
+            void *d = nullptr;
+            BW_CHK(bw_hip_dev_alloc(&d, std::max<size_t>(bytes, 8)));
+            bufs.push_back(d);
+            return static_cast<uint32_t *>(d);
+        };
+        uint32_t *in = alloc(4 * E), *out = alloc(4 * E), *pairs = alloc(8 * E);
+        uint32_t *rin = alloc(4 * E), *rout = alloc(4 * E), *rpairs = alloc(8 * E);
+        BW_CHK(bw_build_edge_lists(V, E, dev.offsets, 4, dev.cols, in, out, pairs, nullptr));
+        BW_CHK(bw_build_edge_lists(V, E, dev.rev_offsets, 4, dev.rev_cols, rin, rout, rpairs, nullptr));
+        BW_CHK(bw_hip_device_sync());
+        dev.el_in = in;
+        dev.el_out = out;
+        dev.sel = pairs;
+        dev.rel_in = rin;
+        dev.rel_out = rout;
+        dev.rsel = rpairs;
+    }
+
     void freeGraph() override
     {
         for (void *p : bufs) (void)bw_hip_dev_free(p);
@@ -362,24 +399,35 @@ struct NativeBFS : public ImplementationBase {
         const uint64_t V = dev.vertex_count;
         if (root >= V) return;
         const uint32_t maxLevels = static_cast<uint32_t>(std::min<uint64_t>(V + 1, 1u << 22));
-        bw_bfs_engine *e = nullptr;
-        BW_CHK(bw_bfs_engine_create(&e, &dev, nullptr, maxLevels));
         const int32_t fixed = nativeVariant(variant);
-        int32_t slen = 1;
+        std::vector<float> w;
+        int32_t outputs = 0;
+        std::vector<int32_t> sched;
+        bool lists = isEdgeListFamily(fixed);
         if (policy == BW_POLICY_LEARNED) {
-            const auto w = readModel(modelPath.empty() ? defaultModelPath() : modelPath, BW_BFS_NATIVE_VARIANTS);
-            BW_CHK(bw_bfs_engine_set_model(e, w.data(), BW_BFS_NATIVE_VARIANTS));
+            w = readModel(modelPath.empty() ? defaultModelPath() : modelPath, &outputs);
+            lists = lists || outputs == BW_BFS_ENGINE_VARIANTS;
         } else if (policy == BW_POLICY_SCHEDULE) {
-            std::vector<int32_t> s;
             size_t start = 0;
             while (start <= schedule.size()) {
                 const size_t comma = schedule.find(',', start);
-                s.push_back(nativeVariant(schedule.substr(start, comma - start)));
+                sched.push_back(nativeVariant(schedule.substr(start, comma - start)));
+                lists = lists || isEdgeListFamily(sched.back());
                 if (comma == std::string::npos) break;
                 start = comma + 1;
             }
-            BW_CHK(bw_bfs_engine_set_schedule(e, s.data(), static_cast<int32_t>(s.size())));
-            slen = static_cast<int32_t>(s.size());
+        }
+        // The edge-list families read edge lists, built on the device from
+        // the CSR (once per graph, outside the timers).
+        if (lists && !dev.el_in) buildEdgeLists();
+        bw_bfs_engine *e = nullptr;
+        BW_CHK(bw_bfs_engine_create(&e, &dev, nullptr, maxLevels));
+        int32_t slen = 1;
+        if (policy == BW_POLICY_LEARNED) {
+            BW_CHK(bw_bfs_engine_set_model(e, w.data(), outputs));
+        } else if (policy == BW_POLICY_SCHEDULE) {
+            BW_CHK(bw_bfs_engine_set_schedule(e, sched.data(), static_cast<int32_t>(sched.size())));
+            slen = static_cast<int32_t>(sched.size());
         }
         Timer computation("computation", run_count);
         std::vector<std::unique_ptr<Timer>> levelTimers;
@@ -419,4 +467,5 @@ extern "C" void registerHIP(Algorithm &result)
     result.addImplementation("native", std::make_unique<NativeBFS>(BW_POLICY_FIXED));
     result.addImplementation("native-schedule", std::make_unique<NativeBFS>(BW_POLICY_SCHEDULE));
     result.addImplementation("native-learned", std::make_unique<NativeBFS>(BW_POLICY_LEARNED));
+    result.addImplementation("native-beamer", std::make_unique<NativeBFS>(BW_POLICY_BEAMER));
 }

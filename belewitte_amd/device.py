@@ -201,6 +201,47 @@ class DeviceGraph:
             setattr(d, ko, a_out.ptr)
         synchronize()
 
+    def relabel_by_degree(self):
+        """The graph relabelled on the device by decreasing out-degree, ties
+        by id (bw_graph_relabel_by_degree): the same graph as
+        Graph.reorder_by_degree() gives on the host, built from the arrays
+        already here with no copy to the host.  Returns (DeviceGraph, perm)
+        with perm (host uint32[V]) the new id of each old vertex."""
+        if self.offset_bytes != 4:
+            raise ValueError("the device relabel reads uint32 offsets")
+        self.ensure("csr")
+        self.ensure("rev_csr")
+        V, E = self.vertex_count, self.edge_count
+        out = DeviceGraph.__new__(DeviceGraph)
+        out.host = None
+        out.vertex_count, out.edge_count, out.offset_bytes, out.undirected = V, E, 4, self.undirected
+        out._a = {}
+        out.desc = N.BwGraphDev()
+        out.desc.vertex_count, out.desc.edge_count, out.desc.offset_bytes = V, E, 4
+        out._a["off"] = DeviceArray(V + 1, np.uint32)
+        out._a["col"] = DeviceArray(max(E, 1), np.uint32)
+        if self.undirected:
+            out._a["roff"], out._a["rcol"] = out._a["off"], out._a["col"]
+        else:
+            out._a["roff"] = DeviceArray(V + 1, np.uint32)
+            out._a["rcol"] = DeviceArray(max(E, 1), np.uint32)
+        perm = DeviceArray(max(V, 1), np.uint32)
+        rev = (None, None, None, None) if self.undirected else (
+            self._a["roff"].ptr, self._a["rcol"].ptr, out._a["roff"].ptr, out._a["rcol"].ptr)
+        try:
+            N.check_hip(N.hip_lib().bw_graph_relabel_by_degree(
+                V, E, self._a["off"].ptr, self._a["col"].ptr, rev[0], rev[1], out._a["off"].ptr,
+                out._a["col"].ptr, rev[2], rev[3], perm.ptr, None))
+            p = perm.download()[:V]
+        except Exception:
+            out.free()
+            raise
+        finally:
+            perm.free()
+        out.desc.offsets, out.desc.cols = out._a["off"].ptr, out._a["col"].ptr
+        out.desc.rev_offsets, out.desc.rev_cols = out._a["roff"].ptr, out._a["rcol"].ptr
+        return out, p
+
     _KEY = {"csr": "off", "rev_csr": "roff", "edge_list": "el_in", "rev_edge_list": "rel_in",
             "struct_edge_list": "sel", "rev_struct_edge_list": "rsel"}
 

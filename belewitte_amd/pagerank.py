@@ -96,13 +96,24 @@ class FusedPageRank:
     Holds the plan and the workspaces, so repeated runs allocate nothing."""
 
     @classmethod
-    def for_graph(cls, g, max_iterations: int = MAX_ITERATIONS, reorder: bool = True):
+    def for_graph(cls, g, max_iterations: int = MAX_ITERATIONS, reorder: bool = True, on_device: bool = True):
         """Uploads `g` for the fused kernel.  reorder=True relabels the
         vertices by decreasing out-degree first (the relabelling of
         reorder-graph.cpp:40-128), which packs the most-gathered
-        contributions into few cache lines; ranks are returned in the
-        original vertex order either way."""
+        contributions into few cache lines: on the device from the uploaded
+        graph (on_device=True), or on the host before the upload.  Ranks are
+        returned in the original vertex order either way."""
         perm = None
+        if reorder and on_device:
+            src = DeviceGraph(g, reps=("csr", "rev_csr"))
+            try:
+                dg, perm = src.relabel_by_degree()
+            finally:
+                src.free()
+            obj = cls(dg, max_iterations)
+            obj.perm = perm
+            obj._owns_dg = True
+            return obj
         if reorder:
             g, perm = g.reorder_by_degree()
         dg = DeviceGraph(g, reps=("rev_csr",))

@@ -218,6 +218,23 @@ int bw_bfs_level(const bw_graph_dev *g, int32_t variant, int32_t reduction,
                  int32_t *levels, int32_t depth, uint32_t *frontier,
                  const bw_launch *cfg);
 
+/* Relabels a graph on the device by decreasing out-degree, ties by id,
+ * the same result as bw_graph_reorder_by_degree computes on the host
+ * (the relabelling of reorder-graph.cpp:40-128): new id i is the i-th
+ * vertex of that order; the rows of each stored direction are permuted and
+ * their column lists mapped and sorted.  Inputs: the CSR (offsets, cols)
+ * and, for a directed graph, the reverse CSR; an undirected graph passes
+ * NULL for rev_offsets, rev_cols, new_rev_offsets and new_rev_cols.
+ * Outputs are caller-allocated device arrays of the input sizes; perm
+ * (device uint32[V], optional) receives the new id of each old vertex.
+ * uint32 offsets, V < 2^31.  Synchronises the stream. */
+int bw_graph_relabel_by_degree(uint64_t vertex_count, uint64_t edge_count,
+                               const uint32_t *offsets, const uint32_t *cols,
+                               const uint32_t *rev_offsets, const uint32_t *rev_cols,
+                               uint32_t *new_offsets, uint32_t *new_cols,
+                               uint32_t *new_rev_offsets, uint32_t *new_rev_cols,
+                               uint32_t *perm, void *stream);
+
 /* Expands the EdgeList / StructEdgeList of GraphLoader.hpp:123-145 from a
  * device CSR on the device: in[j] = row of j, out[j] = cols[j]; pairs gets
  * the interleaved (in, out) form.  Any output may be NULL. */

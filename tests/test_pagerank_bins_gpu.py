@@ -214,3 +214,44 @@ def test_device_plan_ranks_at_scale_20(gpu):
     finally:
         fp.close()
         dg.free()
+
+
+def test_device_relabel_equals_host_relabel(gpu):
+    """bw_graph_relabel_by_degree gives the graph Graph.reorder_by_degree()
+    gives on the host: the same permutation, offsets and sorted column
+    lists in both stored directions; directed and undirected graphs, ties
+    and vertices without edges."""
+    graphs = [graph.rmat(16, 16, seed=61), graph.rmat(15, 8, seed=62, undirected=True), _hub_graph(),
+              graph.erdos_renyi(20000, 90000, seed=63, undirected=False), graph.star(5000), graph.chain(3000),
+              graph.grid2d(100, 70, rewire=0.02, seed=64)]
+    graphs += list(degenerate_graphs())
+    for g in graphs:
+        want, perm_want = g.reorder_by_degree()
+        dg = device.DeviceGraph(g)
+        rg, perm = dg.relabel_by_degree()
+        try:
+            assert np.array_equal(perm, perm_want), g.name
+            for rev, ko, kc in ((False, "off", "col"), (True, "roff", "rcol")):
+                off = rg.array(ko).download()
+                assert np.array_equal(off, want.offsets32(rev)), (g.name, rev)
+                if g.edge_count:
+                    cols = rg.array(kc).download()[:g.edge_count]
+                    assert np.array_equal(cols, want.cols(rev)), (g.name, rev)
+        finally:
+            rg.free()
+            dg.free()
+
+
+def test_device_relabel_pipeline_ranks(gpu):
+    """Relabel and plan both on the device, from the generated order: ranks
+    (returned in the original order) against the oracle."""
+    g = graph.rmat(18, 16, seed=65)
+    csr = O.CSR.from_graph(g)
+    fp = pagerank.FusedPageRank.for_graph(g, max_iterations=20, reorder=True, on_device=True)
+    try:
+        fp.bin(14, 128, 8)
+        for mode, quirk, iters in (("uniform", False, 20), ("drop", True, 20)):
+            ranks, resid, _ = fp.run(iters, mode, nodiv_quirk=quirk)
+            _check(g, csr, ranks, resid, mode, quirk, iters)
+    finally:
+        fp.close()

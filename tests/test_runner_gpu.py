@@ -150,15 +150,22 @@ def test_switch_with_predictor_and_log(gfile, tmp_path):
 
 def test_native_engine_through_the_runner(gfile, tmp_path):
     """The device-driven engine as kernel-runner implementations: fixed
-    variants, a schedule and the learned switch with a model file."""
+    variants (native and reference families), a schedule, Beamer's rule
+    and the learned switch with an 11-output and a 3-output model file."""
     from belewitte_amd import bfs_native, switch
     g, p = gfile
     root = g.max_out_degree_vertex()
     want, nlev = O.bfs_queue(O.CSR.from_graph(g), root)
-    model = tmp_path / "m.json"
-    switch.Model.init(np.random.default_rng(0).normal(8, 3, (32, bfs_native.NUM_FEATURES)), seed=1, target_mean=9.0).save(model)
+    X = np.random.default_rng(0).normal(8, 3, (32, bfs_native.NUM_FEATURES))
+    model, model3 = tmp_path / "m.json", tmp_path / "m3.json"
+    switch.Model.init(X, seed=1, target_mean=9.0).save(model)
+    switch.Model.init(X, outputs=3, seed=2, target_mean=9.0).save(model3)
     jobs = [("native", ["--variant", "td"]), ("native", ["--variant", "bu-wave"]),
-            ("native-schedule", ["--schedule", "bu,td,bu-wave"]), ("native-learned", ["--model", str(model)])]
+            ("native", ["--variant", "edge-list"]), ("native", ["--variant", "vertex-pull-warp"]),
+            ("native-schedule", ["--schedule", "bu,td,rev-struct-edge-list,bu-wave"]),
+            ("native-beamer", []),
+            ("native-learned", ["--model", str(model)]), ("native-learned", ["--model", str(model3)]),
+            ("native-learned", [])]
     for i, (name, extra) in enumerate(jobs):
         out = tmp_path / str(i)
         out.mkdir()
