@@ -35,7 +35,7 @@ def test_bfs_implementation_names():
     r = run("list", "implementations", "-a", "bfs")
     assert r.returncode == 0
     assert r.stdout.split() == sorted(list(bfs.IMPLEMENTATIONS) + ["switch", "native", "native-learned",
-                                                                   "native-schedule"])
+                                                                   "native-schedule", "native-beamer"])
 
 
 def test_pagerank_implementation_names():
