@@ -33,6 +33,7 @@
 #include "pagerank_plan.hpp"
 
 #include <algorithm>
+#include <cmath>
 #include <thread>
 #include <vector>
 
@@ -471,12 +472,20 @@ __global__ __launch_bounds__(kBlock * kWinTiles) void pr_windows_kernel(IterArgs
 // per iteration an accumulate launch (chunks of one partition's edges into
 // an LDS row vector, written out as one partial per chunk) and a finalise
 // launch (partials summed in chunk order into new ranks, whose
-// contributions are scattered to the bins of the next iteration).  The LDS
-// row accumulators are FP64 (ds_add_f64): their order of addition within a
-// chunk changes from run to run, but each partial is rounded to FP32 from
-// an FP64 sum, so the ranks practically never do.  (FP32 LDS atomics were
-// measured 3.4x slower than FP64 ones in this kernel on gfx950, besides
-// drifting on hub rows.)
+// contributions are scattered to the bins of the next iteration).
+//
+// The LDS row accumulators are 64-bit fixed point (ds_add_u64): a bin holds
+// its contribution c (the FP32 rank / out-degree) scaled by 2^(S - 32), and
+// the accumulate adds floor(c * 2^S) as an integer, split into the integer
+// and fraction parts of the stored value.  Integer addition is exact and
+// associative, so a chunk's row sums do not depend on the order of the
+// atomics and the ranks repeat bit for bit; each sum is below the exact one
+// by less than (edges of the row in the chunk) * 2^-S, and is rounded once
+// to the FP32 partial.  S = 62 - ceil(log2 M) with M the bound on a row sum
+// (the rank mass: 1, or 1 + E/V after the undivided first iteration of the
+// nodiv quirk), so no sum reaches 2^63.  On gfx950 LDS atomics on random
+// rows of a 2^14-row table run at 4.3 per CU and clock as ds_add_u64 (the
+// rate of plain 64-bit stores), 2.5 as ds_add_f64 and 0.33 as ds_add_f32.
 
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4u __attribute__((ext_vector_type(4))) __attribute__((aligned(4)));  // 4-byte aligned
@@ -489,7 +498,7 @@ struct BinArgs {
     const uint4 *chunks;        // (task begin, task end, partition, slot)
     uint32_t nchunks;
     uint32_t npairs;            // length of bins (at least 1)
-    float *bins;                // one contribution per pair
+    float *bins;                // one contribution per pair, times pair_scale
     float *partials;            // [slot][rows per partition]
     const uint32_t *chunk_base; // per partition: its first slot (K + 1)
     const uint4 *blocks;        // (first vertex, end vertex, run begin, run count)
@@ -509,7 +518,31 @@ struct BinArgs {
     int phase;    // 0: initial ranks 1/V; 1: finalise from the partials
     int scatter;  // write the bins for the next iteration
     int divide;   // phase 0: contributions divided by the out-degree
+    float pair_scale;     // 2^(S - 32): a bin holds contribution * pair_scale
+    float partial_scale;  // 2^-S: a fixed-point row sum times this is its value
 };
+
+// Fixed-point value floor(y * 2^32) of a stored bin value y (0 <= y < 2^32).
+__device__ __forceinline__ unsigned long long bin_fixed(float y)
+{
+    const uint32_t hi = static_cast<uint32_t>(y);
+    const uint32_t lo = static_cast<uint32_t>(__builtin_amdgcn_fractf(y) * 4294967296.0f);
+    return (static_cast<unsigned long long>(hi) << 32) | lo;
+}
+
+// The fixed-point exponent S of the binned iteration: a row sum never
+// exceeds the rank mass M (1; 1 + E/V when the first iteration is not
+// divided by the out-degree), and S = 62 - ceil(log2 M) keeps every sum
+// and every stored value y = c * 2^(S - 32) below 2^31.
+inline int bins_fixed_shift(uint64_t V, uint64_t E, bool undivided_first)
+{
+    int lg = 0;
+    if (undivided_first) {
+        const double M = 1.0 + static_cast<double>(E) / static_cast<double>(V);
+        while (std::ldexp(1.0, lg) < M) lg++;
+    }
+    return 62 - lg;
+}
 
 // Workgroup per chunk; the last workgroup reduces the previous finalise's
 // (residual, dangling) partials.  Each wave takes whole tasks; per round a
@@ -526,7 +559,7 @@ __global__ __launch_bounds__(kThreads, kRowsLog == 13 ? 8 : 4) void pr_bins_accu
     constexpr uint32_t kRound = kWave * kLaneEdges;      // edges per wave and round
     constexpr int kCountBits = kLaneEdges == 8 ? 4 : 5;  // flag count 0..kLaneEdges
     static_assert(kWaves == static_cast<int>(bins::kChunkWaves), "the plan makes one task per wave");
-    __shared__ double acc[kRows];
+    __shared__ unsigned long long acc[kRows];
     __shared__ uint4 stask[bins::kChunkWaves];  // the chunk's task records
     __shared__ double red[kWaves][2];
     if (blockIdx.x == gridDim.x - 1) {
@@ -557,7 +590,7 @@ __global__ __launch_bounds__(kThreads, kRowsLog == 13 ? 8 : 4) void pr_bins_accu
     };
     for (uint32_t c = blockIdx.x; c < a.nchunks; c += gridDim.x - 1) {
         const uint4 ch = a.chunks[c];
-        for (uint32_t i = threadIdx.x; i < kRows; i += kThreads) acc[i] = 0.0;
+        for (uint32_t i = threadIdx.x; i < kRows; i += kThreads) acc[i] = 0ull;
         // Task records come from LDS in the loop: a global load there would
         // make every wait for it a wait for all the loads in flight.
         for (uint32_t i = threadIdx.x; i < ch.y - ch.x; i += kThreads) stask[i] = a.tasks[ch.x + i];
@@ -701,7 +734,7 @@ __global__ __launch_bounds__(kThreads, kRowsLog == 13 ? 8 : 4) void pr_bins_accu
 #pragma unroll
                         for (int i = 1; i <= j; i++) x = o == static_cast<uint32_t>(i) ? vv[i] : x;
                     }
-                    if (j < static_cast<int>(nn)) atomicAdd(&acc[code_of(cd, j) & (kRows - 1)], static_cast<double>(x));
+                    if (j < static_cast<int>(nn)) atomicAdd(&acc[code_of(cd, j) & (kRows - 1)], bin_fixed(x));
                 }
             };
             // kDepth rounds of pair values in flight.  Rounds R_0, R_1, ... of
@@ -744,7 +777,8 @@ __global__ __launch_bounds__(kThreads, kRowsLog == 13 ? 8 : 4) void pr_bins_accu
         }
         __syncthreads();
         float *out = a.partials + static_cast<size_t>(ch.w) * kRows;
-        for (uint32_t i = threadIdx.x; i < kRows; i += kThreads) out[i] = static_cast<float>(acc[i]);
+        for (uint32_t i = threadIdx.x; i < kRows; i += kThreads)
+            out[i] = static_cast<float>(acc[i]) * a.partial_scale;
         __syncthreads();  // acc is cleared for the next chunk
     }
 }
@@ -809,7 +843,7 @@ __global__ __launch_bounds__(kThreads) void pr_bins_finalise_kernel(BinArgs a)
                 }
                 if (dv == 0) dangl += static_cast<double>(nr);
                 a.rank[v] = nr;
-                contrib[vi] = cv;
+                contrib[vi] = cv * a.pair_scale;
             }
         }
         if (a.scatter) {
@@ -1393,6 +1427,9 @@ int run_binned(const bw_pr_plan *plan, int32_t iterations, int32_t mode, int32_t
     a.phase = 0;
     a.scatter = 1;
     a.divide = !(nodiv_quirk && mode == BW_PR_DANGLING_DROP);
+    const int S = bins_fixed_shift(plan->V, plan->E, !a.divide);
+    a.pair_scale = std::ldexp(1.0f, S - 32);
+    a.partial_scale = std::ldexp(1.0f, -S);
     const uint32_t finGrid = B.nblocks, accGrid = B.nchunks + 1;
     hipLaunchKernelGGL(finK, dim3(finGrid), dim3(kBinFinThreads), 0, s, a);
     BW_LAUNCH_CHECK();

@@ -420,9 +420,9 @@ struct PageRank : public ImplementationTemplate<Platform, Vertex, Edge, switchin
 // DROP mode keeps the reference's undivided first iteration, UNIFORM mode
 // redistributes the dangling mass.  Same lifecycle, --iterations and
 // --dangling flags and output formats as the implementations above.  By
-// default the vertices are relabelled by decreasing out-degree on upload
-// (--no-reorder keeps the file's order; the output is in the file's order
-// either way) and each iteration is the binned form (propagation blocking):
+// default the vertices are relabelled by decreasing out-degree on the
+// device after the upload (bw_graph_relabel_by_degree; --no-reorder keeps
+// the file's order; the output is in the file's order either way) and each iteration is the binned form (propagation blocking):
 // partitions of 2^14 rows (--bins 13/14; --bins 0 selects the pull
 // kernels), chunks of 128 tasks of 2048 edges (--bin-chunk N), 8 edges per
 // lane.  A graph the binned layout cannot hold (more than 2048 partitions,
@@ -468,31 +468,51 @@ struct FusedPageRank : public ImplementationBase {
         return d;
     }
 
+    // Frees one buffer of upload() before freeGraph().
+    void release(void *p)
+    {
+        auto it = std::find(bufs.begin(), bufs.end(), p);
+        if (it == bufs.end()) return;
+        bufs.erase(it);
+        BW_CHK(bw_hip_dev_free(p));
+    }
+
     void loadGraph(const std::string file) override
     {
         freeGraph();
         if (bw_graph_load(&host, file.c_str()) != 0) reportError("Failed to load '", file, "': ", bw_graph_last_error());
-        const uint64_t V = bw_graph_vertex_count(host);
-        const bw_graph *g = host;
-        bw_graph *relabelled = nullptr;
-        if (!keepOrder) {
-            perm.assign(V, 0);
-            if (bw_graph_reorder_by_degree(&relabelled, host, perm.data()) != 0) reportError(bw_graph_last_error());
-            g = relabelled;
-        } else {
-            perm.clear();
-        }
-        // Sizes come from the graph that is uploaded (the relabelling keeps
-        // every stored edge, so its count equals the file's).
-        const uint64_t E = bw_graph_edge_count(g);
+        const uint64_t V = bw_graph_vertex_count(host), E = bw_graph_edge_count(host);
         std::vector<uint32_t> off(V + 1);
-        if (bw_graph_offsets32(g, 1, off.data()) != 0) reportError(bw_graph_last_error());
+        if (bw_graph_offsets32(host, 1, off.data()) != 0) reportError(bw_graph_last_error());
         dev.vertex_count = V;
         dev.edge_count = E;
         dev.offset_bytes = 4;
-        dev.rev_offsets = upload(off.data(), 4 * (V + 1));
-        dev.rev_cols = static_cast<uint32_t *>(upload(bw_graph_cols(g, 1), 4 * E));
-        if (relabelled) bw_graph_free(relabelled);
+        void *roff = upload(off.data(), 4 * (V + 1));
+        void *rcols = upload(bw_graph_cols(host, 1), 4 * E);
+        if (keepOrder || V == 0) {  // (no vertex: the plan refuses it later)
+            perm.clear();
+            dev.rev_offsets = roff;
+            dev.rev_cols = static_cast<uint32_t *>(rcols);
+            return;
+        }
+        // The relabelling reads both directions (a symmetric graph's two
+        // are equal and relabel alike); only the new reverse CSR is kept.
+        if (bw_graph_offsets32(host, 0, off.data()) != 0) reportError(bw_graph_last_error());
+        void *foff = upload(off.data(), 4 * (V + 1));
+        void *fcols = upload(bw_graph_cols(host, 0), 4 * E);
+        void *nfoff = upload(nullptr, 4 * (V + 1)), *nfcols = upload(nullptr, 4 * E);
+        void *nroff = upload(nullptr, 4 * (V + 1)), *nrcols = upload(nullptr, 4 * E);
+        void *dperm = upload(nullptr, 4 * V);
+        BW_CHK(bw_graph_relabel_by_degree(V, E, static_cast<const uint32_t *>(foff), static_cast<const uint32_t *>(fcols),
+                                          static_cast<const uint32_t *>(roff), static_cast<const uint32_t *>(rcols),
+                                          static_cast<uint32_t *>(nfoff), static_cast<uint32_t *>(nfcols),
+                                          static_cast<uint32_t *>(nroff), static_cast<uint32_t *>(nrcols),
+                                          static_cast<uint32_t *>(dperm), nullptr));
+        perm.assign(V, 0);
+        if (V) BW_CHK(bw_hip_memcpy_d2h(perm.data(), dperm, 4 * V));
+        for (void *p : {roff, rcols, foff, fcols, nfoff, nfcols, dperm}) release(p);
+        dev.rev_offsets = nroff;
+        dev.rev_cols = static_cast<uint32_t *>(nrcols);
     }
 
     void freeGraph() override

@@ -5,8 +5,9 @@ vertex-pull-nodiv (pagerank/vertex_pull_nodiv.cu:3-29 with
 consolidateRankNoDiv, pagerank/pagerank.cu:142-165) and the FP64
 recurrence.  Bar: 1e-6 max-abs against the FP32 oracle, widened by the
 FP32 oracle's own error against FP64 (its sequential sums drift on hub
-rows), and 1e-6 against the FP64 recurrence; the FP64 row accumulators
-add in a run-dependent order, so ranks repeat to FP32 rounding.
+rows), and 1e-6 against the FP64 recurrence.  The row accumulators are
+64-bit fixed point (integer sums, independent of the order of the
+atomics), so the ranks repeat bit for bit from run to run.
 """
 import numpy as np
 import pytest
@@ -70,8 +71,9 @@ def test_binned_matches_oracle(cases, rows_log2, chunk_tasks, lanes, mode, quirk
         # The residuals follow the pull kernel's (Kahan-compensated) ones to
         # FP32 rounding.
         assert np.allclose(resid, rp, rtol=1e-3, atol=1e-6), (g.name, resid[:4], rp[:4])
-        again, _, _ = fp.run(iters, mode, nodiv_quirk=quirk)
-        assert float(np.max(np.abs(again - ranks))) <= 4e-7 * float(ranks.max())
+        again, ragain, _ = fp.run(iters, mode, nodiv_quirk=quirk)
+        assert np.array_equal(again.view(np.uint32), ranks.view(np.uint32)), g.name
+        assert np.array_equal(ragain, resid), g.name
         # Back to the pull kernels: the same ranks as before, bit for bit.
         fp.bin(0)
         back, _, _ = fp.run(iters, mode, nodiv_quirk=quirk)
@@ -110,6 +112,33 @@ def test_binned_degenerate(gpu, mode):
         assert np.all(np.isfinite(resid))
         fp.close()
         dg.free()
+
+
+@pytest.mark.parametrize("mode,quirk", [("drop", True), ("drop", False), ("uniform", False)])
+def test_binned_dense_multigraph(gpu, tmp_path, mode, quirk):
+    """16 vertices and 60,000 edges, most of them repeated into vertex 0 (a
+    .graph file keeps repeated edges): after the undivided first iteration
+    of the nodiv quirk that row's sum is about 3,200, far above the rank
+    mass of 1, which the fixed-point row accumulators must hold without
+    overflow."""
+    from graphfiles import write_v1
+    rng = np.random.default_rng(12)
+    src = rng.integers(0, 16, 60000)
+    dst = np.where(rng.random(60000) < 0.85, 0, rng.integers(0, 16, 60000))
+    write_v1(tmp_path / "multi.graph", 16, src, dst)
+    g = graph.load(tmp_path / "multi.graph")
+    assert g.edge_count == 60000
+    csr = O.CSR.from_graph(g)
+    dg = device.DeviceGraph(g)
+    fp = pagerank.FusedPageRank(dg, max_iterations=12)
+    fp.bin(13, 1, 8)
+    for iters in (1, 2, 12):
+        ranks, resid, _ = fp.run(iters, mode, nodiv_quirk=quirk)
+        x = O.pagerank_f64(csr, iters, mode, nodiv_quirk=quirk)
+        assert float(np.max(np.abs(ranks - x) / x)) <= 1e-6, (mode, quirk, iters, ranks[:4], x[:4])
+        assert np.all(np.isfinite(resid))
+    fp.close()
+    dg.free()
 
 
 def test_binned_rejects_bad_settings(gpu):
