@@ -790,10 +790,13 @@ __global__ __launch_bounds__(kThreads, kRowsLog == 13 ? 8 : 4) void pr_bins_accu
 // by a group of G lanes (G a power of two, as many as the block's vertex
 // count leaves, at most one per partial), lane i adding partials i, i + G,
 // ... in order and the group combining by a fixed xor tree.
+#ifndef BW_BINS_SCATTER_UNROLL
+#define BW_BINS_SCATTER_UNROLL 8  // pair stores per lane and step of the scatter
+#endif
 template<int kRowsLog, int kThreads>
 __global__ __launch_bounds__(kThreads) void pr_bins_finalise_kernel(BinArgs a)
 {
-    constexpr int kScatterUnroll = 8;
+    constexpr int kScatterUnroll = BW_BINS_SCATTER_UNROLL;
     constexpr uint32_t kWaves = kThreads / kWave;
     const uint32_t lane = threadIdx.x & (kWave - 1);
     const uint32_t wid = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
@@ -1384,7 +1387,10 @@ constexpr int kBinAccThreads = 1024;
 #ifndef BW_BINS_DEPTH14
 #define BW_BINS_DEPTH14 2
 #endif
-constexpr int kBinFinThreads = 512;
+#ifndef BW_BINS_FIN_THREADS
+#define BW_BINS_FIN_THREADS 512
+#endif
+constexpr int kBinFinThreads = BW_BINS_FIN_THREADS;
 
 // The binned iteration loop: initial ranks and bins, then per iteration one
 // accumulate and one finalise launch; ranks are updated in place.

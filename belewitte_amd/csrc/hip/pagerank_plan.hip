@@ -10,8 +10,10 @@
 //      source's rows come in increasing order;
 //   2. the pairs: the first edge of each (source, partition) run, compacted
 //      by an exclusive scan of the new-pair flags;
-//   3. the scatter blocks: the host's greedy cut, one wave per partition,
-//      vertex after vertex with the current block's partition set in LDS;
+//   3. the scatter blocks: the host's greedy cut, one wave per partition
+//      walking its vertices with wave-uniform state, then each block's runs
+//      (distinct target partitions) counted in an LDS bitmap, one
+//      workgroup per block;
 //   4. the bin index of each pair: a stable sort of the pairs by partition;
 //      the scatter (flat) order: a stable sort by (block, partition);
 //   5. the edge codes: the forward edges stably sorted by partition.
@@ -36,8 +38,6 @@ namespace {
 
 constexpr uint32_t kThreads = 256;
 constexpr uint32_t kEdgesPerStep = 4096;  // edges per workgroup step of the row search
-constexpr uint32_t kStageVerts = 64;      // vertices the greedy stages per round
-constexpr uint32_t kStagePairs = 4096;    // pairs it stages in LDS per round (more: read from memory)
 constexpr uint32_t kTouchedWords = kMaxRuns / 32;
 
 uint32_t grid_for(uint64_t n, uint32_t per = kThreads)
@@ -165,80 +165,77 @@ __global__ void plan_bounds_kernel(uint32_t V, uint32_t L, uint32_t K, const uin
 
 // The scatter blocks of partition k = blockIdx.x, cut like the host
 // builder: consecutive vertices, a new block when the current one holds
-// kBlockVerts vertices, or when adding the vertex would take the block's
-// work (vertices x chunks of the partition + pairs) past kBlockWork or its
-// distinct target partitions past kMaxRuns.  One wave; every decision is
-// wave-uniform.  Writes each vertex's block (local to the partition) and,
-// at k << L + b, block b's (first vertex, end vertex, pairs, runs).
-__global__ __launch_bounds__(64) void plan_blocks_kernel(uint32_t V, uint32_t L, const uint32_t *pbase,
-                                                         const uint32_t *pair_part, const uint32_t *chunk_base,
-                                                         uint32_t *block_local, uint4 *pblocks, uint32_t *nblocks)
+// kBlockVerts vertices or when adding the vertex would take the block's
+// work (vertices x chunks of the partition + pairs) past kBlockWork.  (The
+// host builder also closes a block whose distinct target partitions would
+// pass kMaxRuns; that never happens, since a plan has at most kMaxRuns
+// partitions.)  One wave: lane j holds the pair count of vertex base + j,
+// and the greedy walks them with wave-uniform state.  Writes each vertex's
+// block (local to the partition) and, at k << L + b, block b's (first
+// vertex, end vertex, pairs, 0); plan_block_runs_kernel fills in the runs.
+__global__ __launch_bounds__(64) void plan_cut_kernel(uint32_t V, uint32_t L, const uint32_t *pbase,
+                                                      const uint32_t *chunk_base, uint32_t *block_local,
+                                                      uint4 *pblocks, uint32_t *nblocks)
 {
-    __shared__ uint32_t touched[kTouchedWords];
-    __shared__ uint32_t sb[kStageVerts + 1];
-    __shared__ uint32_t sp[kStagePairs];
     const uint32_t k = blockIdx.x, lane = threadIdx.x;
     const uint32_t vBeg = k << L;
     const uint32_t vEnd = static_cast<uint32_t>(std::min<uint64_t>(static_cast<uint64_t>(vBeg) + (1u << L), V));
     const uint64_t perVertex = std::max<uint32_t>(1, chunk_base[k + 1] - chunk_base[k]);
-    for (uint32_t i = lane; i < kTouchedWords; i += 64) touched[i] = 0;
-    uint32_t nb = 0, v0 = vBeg, ntouched = 0, npairs = 0;
+    uint32_t nb = 0, v0 = vBeg, npairs = 0;
     uint64_t work = 0;
     auto close_block = [&](uint32_t end) {
-        if (lane == 0) pblocks[vBeg + nb] = make_uint4(v0, end, npairs, ntouched);
+        if (lane == 0) pblocks[vBeg + nb] = make_uint4(v0, end, npairs, 0u);
         nb++;
         v0 = end;
         work = 0;
         npairs = 0;
-        ntouched = 0;
-        __syncthreads();
-        for (uint32_t i = lane; i < kTouchedWords; i += 64) touched[i] = 0;
-        __syncthreads();
     };
-    for (uint32_t base = vBeg; base < vEnd; base += kStageVerts) {
-        const uint32_t cnt = std::min(kStageVerts, vEnd - base);
-        __syncthreads();
-        if (lane < cnt) sb[lane] = pbase[base + lane];
-        if (lane == 0) sb[cnt] = pbase[base + cnt];
-        __syncthreads();
-        const uint32_t p0 = sb[0], np = sb[cnt] - p0;
-        const bool staged = np <= kStagePairs;
-        if (staged)
-            for (uint32_t i = lane; i < np; i += 64) sp[i] = pair_part[p0 + i];
-        __syncthreads();
+    for (uint32_t base = vBeg; base < vEnd; base += 64) {
+        const uint32_t s = base + lane, cnt = std::min(64u, vEnd - base);
+        const uint32_t n = s < vEnd ? pbase[s + 1] - pbase[s] : 0u;
+        uint32_t mine = 0;  // the block of vertex s
         for (uint32_t j = 0; j < cnt; j++) {
-            const uint32_t s = base + j;
-            const uint32_t a = sb[j] - p0, n = sb[j + 1] - sb[j];
-            if (s - v0 >= kBlockVerts) close_block(s);
-            // Partitions of s's pairs not yet in the block.
-            uint32_t fresh = 0;
-            for (uint32_t i0 = 0; i0 < n; i0 += 64) {
-                const uint32_t i = i0 + lane;
-                bool f = false;
-                if (i < n) {
-                    const uint32_t kk = staged ? sp[a + i] : pair_part[p0 + a + i];
-                    f = !((touched[kk >> 5] >> (kk & 31)) & 1u);
-                }
-                fresh += static_cast<uint32_t>(__popcll(__ballot(f)));
-            }
-            const uint64_t w = perVertex + n;
-            if (s > v0 && (work + w > kBlockWork || ntouched + fresh > kMaxRuns)) {
-                close_block(s);
-                fresh = n;  // the new block holds no partition yet; s's pairs have distinct ones
-            }
-            for (uint32_t i = lane; i < n; i += 64) {
-                const uint32_t kk = staged ? sp[a + i] : pair_part[p0 + a + i];
-                atomicOr(&touched[kk >> 5], 1u << (kk & 31));
-            }
-            __syncthreads();
-            ntouched += fresh;
-            npairs += n;
+            const uint32_t sj = base + j, nj = __builtin_amdgcn_readlane(n, j);
+            if (sj - v0 >= kBlockVerts) close_block(sj);
+            const uint64_t w = perVertex + nj;
+            if (sj > v0 && work + w > kBlockWork) close_block(sj);
+            npairs += nj;
             work += w;
-            if (lane == 0) block_local[s] = nb;
+            if (lane == j) mine = nb;
         }
+        if (s < vEnd) block_local[s] = mine;
     }
     close_block(vEnd);
     if (lane == 0) nblocks[k] = nb;
+}
+
+// Runs of scatter block blockIdx.x: the distinct partitions its pairs
+// point into, counted in an LDS bitmap (a plan has at most kMaxRuns
+// partitions).  Written to the block record and to bruns.
+__global__ __launch_bounds__(kThreads) void plan_block_runs_kernel(const uint32_t *pbase, const uint32_t *pair_part,
+                                                                   uint4 *blocks, uint32_t *bruns)
+{
+    __shared__ uint32_t touched[kTouchedWords];
+    __shared__ uint32_t total;
+    const uint32_t b = blockIdx.x;
+    for (uint32_t i = threadIdx.x; i < kTouchedWords; i += kThreads) touched[i] = 0;
+    if (threadIdx.x == 0) total = 0;
+    __syncthreads();
+    const uint4 r = blocks[b];
+    const uint32_t p1 = pbase[r.y];
+    for (uint32_t p = pbase[r.x] + threadIdx.x; p < p1; p += kThreads) {
+        const uint32_t kk = pair_part[p];
+        atomicOr(&touched[kk >> 5], 1u << (kk & 31));
+    }
+    __syncthreads();
+    uint32_t c = 0;
+    for (uint32_t i = threadIdx.x; i < kTouchedWords; i += kThreads) c += static_cast<uint32_t>(__popc(touched[i]));
+    if (c) atomicAdd(&total, c);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        blocks[b].w = total;
+        bruns[b] = total;
+    }
 }
 
 // Global block records from the per-partition ones: (first vertex, end
@@ -561,8 +558,8 @@ int build_plan_device(const uint32_t *roff, const uint32_t *rcols, uint32_t V, u
     PLAN_ALLOC(pblocks, uint4, V);
     PLAN_ALLOC(nblocksK, uint32_t, K);
     PLAN_TRY(hipMemcpyAsync(dchunk_base, chunk_base.data(), sizeof(uint32_t) * (K + 1), hipMemcpyHostToDevice, s));
-    hipLaunchKernelGGL(plan_blocks_kernel, dim3(K), dim3(64), 0, s, V, L, pbase, pair_part, dchunk_base, block_local,
-                       pblocks, nblocksK);
+    hipLaunchKernelGGL(plan_cut_kernel, dim3(K), dim3(64), 0, s, V, L, pbase, dchunk_base, block_local, pblocks,
+                       nblocksK);
     PLAN_TRY(hipGetLastError());
     std::vector<uint32_t> hnb(K), block_base(K + 1, 0);
     PLAN_TRY(hipMemcpyAsync(hnb.data(), nblocksK, sizeof(uint32_t) * K, hipMemcpyDeviceToHost, s));
@@ -583,6 +580,10 @@ int build_plan_device(const uint32_t *roff, const uint32_t *rcols, uint32_t V, u
     hipLaunchKernelGGL(plan_gather_blocks_kernel, dim3(std::min<uint32_t>(K, 1024)), dim3(kThreads), 0, s, K, L,
                        dblock_base, pblocks, blocks, bpairs, bruns);
     PLAN_TRY(hipGetLastError());
+    if (NB) {
+        hipLaunchKernelGGL(plan_block_runs_kernel, dim3(NB), dim3(kThreads), 0, s, pbase, pair_part, blocks, bruns);
+        PLAN_TRY(hipGetLastError());
+    }
     PLAN_TRY(hipMemsetAsync(bpairs + NB, 0, sizeof(uint32_t), s));
     PLAN_TRY(hipMemsetAsync(bruns + NB, 0, sizeof(uint32_t), s));
     PLAN_TRY(exclusive_scan(sc, bpairs, pair_scan_b, static_cast<uint64_t>(NB) + 1, s));

@@ -589,6 +589,12 @@ def main() -> int:
     out["natural_order"] = {"ge_s_iter": round(E / nat_s / 1e9, 3), "avg_launch_us": round(nat_s * 1e6, 2),
                             "frac": round(bytes_per_launch / nat_s / 1e9 / HBM_PEAK_GBS, 4),
                             "shape": natural.kernel_names()}
+    if natural.binning and fused.binning:
+        # The binned layout's size follows the vertex order: a pair is a
+        # (source, partition) with an edge between them, and the relabel puts
+        # the most-gathered vertices into few partitions.
+        out["natural_order"]["pairs"] = natural.bin_info()["pairs"]
+        out["natural_order"]["pairs_relabelled"] = fused.bin_info()["pairs"]
     natural.close()
 
     if not args.no_variants:
