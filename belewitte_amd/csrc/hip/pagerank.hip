@@ -43,10 +43,19 @@ __device__ __forceinline__ void add_f64(double *p, double v) { unsafeAtomicAdd(p
 // and adds every run of equal values inside them with one atomic from the
 // run's last lane: a source-sorted edge list, whose lanes would otherwise
 // all add to one counter, takes an atomic per run instead of per edge.
+// Values below kHotIds are counted in LDS and added to deg once per
+// workgroup: a graph relabelled by degree has its hubs there, whose
+// counters would otherwise take one global atomic per edge, one after the
+// other.
+constexpr uint32_t kHotIds = 16384;
+
 template<int STRIDE>
 __global__ __launch_bounds__(256) void pr_degrees_runs_kernel(uint64_t E, const uint32_t *__restrict__ values,
                                                               int off, uint32_t *deg)
 {
+    __shared__ uint32_t hot[kHotIds];
+    for (uint32_t i = threadIdx.x; i < kHotIds; i += blockDim.x) hot[i] = 0;
+    __syncthreads();
     const uint32_t lane = threadIdx.x & (kWave - 1);
     const uint64_t waves = static_cast<uint64_t>(gridDim.x) * (blockDim.x / kWave);
     for (uint64_t w = (static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x) / kWave; w * kWave < E;
@@ -60,8 +69,23 @@ __global__ __launch_bounds__(256) void pr_degrees_runs_kernel(uint64_t E, const 
         const uint64_t heads = __ballot(head);
         const uint64_t upto = lane == kWave - 1 ? ~0ull : ((2ull << lane) - 1ull);
         const uint32_t h = 63u - static_cast<uint32_t>(__builtin_clzll(heads & upto));
-        if (valid && tail) atomicAdd(&deg[x], lane - h + 1u);
+        if (valid && tail) {
+            if (x < kHotIds) atomicAdd(&hot[x], lane - h + 1u);
+            else atomicAdd(&deg[x], lane - h + 1u);
+        }
     }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < kHotIds; i += blockDim.x)
+        if (hot[i]) atomicAdd(&deg[i], hot[i]);
+}
+
+// Workgroups of the degree count: two per CU (the LDS counters take 64 KB),
+// fewer for a short list, so that each flushes its counters after a long
+// stretch of entries.
+inline uint32_t degree_grid(uint64_t E)
+{
+    const uint64_t need = (E + 255) / 256;
+    return static_cast<uint32_t>(std::max<uint64_t>(1, std::min<uint64_t>(need, 2ull * device_cus())));
 }
 
 template<typename Off>
@@ -813,8 +837,7 @@ extern "C" int bw_pr_degrees_edge_list(uint64_t E, const uint32_t *origins, uint
 {
     if (E == 0) return BW_HIP_OK;
     if (!origins || !deg) return fail(BW_HIP_ERR_ARG, "NULL device pointer");
-    uint32_t block = 256, grid = 0;
-    BW_TRY(resolve_geometry(nullptr, E, block, grid));
+    const uint32_t block = 256, grid = degree_grid(E);
     hipLaunchKernelGGL(pr_degrees_runs_kernel<1>, dim3(grid), dim3(block), 0, stream_of(cfg), E, origins, 0, deg);
     BW_LAUNCH_CHECK();
     return BW_HIP_OK;
@@ -825,8 +848,7 @@ extern "C" int bw_pr_degrees_struct_edge_list(uint64_t E, const uint32_t *edges,
 {
     if (E == 0) return BW_HIP_OK;
     if (!edges || !deg) return fail(BW_HIP_ERR_ARG, "NULL device pointer");
-    uint32_t block = 256, grid = 0;
-    BW_TRY(resolve_geometry(nullptr, E, block, grid));
+    const uint32_t block = 256, grid = degree_grid(E);
     hipLaunchKernelGGL(pr_degrees_runs_kernel<2>, dim3(grid), dim3(block), 0, stream_of(cfg), E, edges,
                        reverse ? 1 : 0, deg);
     BW_LAUNCH_CHECK();
