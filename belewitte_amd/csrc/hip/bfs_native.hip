@@ -73,7 +73,7 @@ struct BfsState {
     double frontier_sq;         // sum of squared out-degrees of the frontier
     uint64_t t_step;            // wall clock at the start of the level's step kernel
     uint32_t fill_levels;       // the previous level left its depths to this one (dense top-down)
-    uint32_t pad_;
+    uint32_t stalled;           // a level's variant was of a kind that was not launched (see run_impl)
     uint64_t explored_edges;    // out-degree sum of every vertex discovered so far
     uint64_t prev_frontier;     // frontier size of the previous level (0 at level 0)
 };
@@ -828,6 +828,14 @@ __global__ __launch_bounds__(kBlock) void bfs_engine_epilogue(Graph g, Buffers b
     if (st.done) return;
     const uint64_t now = wall_clock64();
     const int tid = threadIdx.x, lane = tid & (kWave - 1), wid = tid / kWave;
+    // The step kernel that ran the level stamped t_step; none did when the
+    // level's variant is of a kind this launch left out.  Then the level is
+    // still pending: the state stays as it is, and the host relaunches it
+    // with both kinds.
+    if (st.t_step == 0) {
+        if (tid == 0) st.stalled = 1;
+        return;
+    }
     LaneModel lm;
     if (wid == 0) load_lane_model(p, lm);
     uint64_t os = 0;
@@ -889,6 +897,7 @@ __global__ __launch_bounds__(kBlock) void bfs_engine_epilogue(Graph g, Buffers b
         // Level time: from the start of the step kernel's first workgroup
         // to the start of this epilogue.
         const float ns = static_cast<float>(now - st.t_step) * b.ns_per_tick;
+        st.t_step = 0;  // read above by every thread, before the barriers
         if (d < b.max_levels) {
             b.counts[d] = nnew;
             b.times[d] = static_cast<uint64_t>(ns);
@@ -960,6 +969,7 @@ __global__ __launch_bounds__(kBlock) void bfs_engine_init(Graph g, Buffers b, Po
         st.prev_choice = -1;
         st.prev_time_ns = 0.0f;
         st.t_step = 0;
+        st.stalled = 0;
         st.fill_levels = 0;
         st.qcount = nq;
         st.hubcount = nh;
@@ -1114,9 +1124,8 @@ int launch_step(bw_bfs_engine *e, bool ref, hipStream_t s)
     return BW_HIP_OK;
 }
 
-int launch_level(bw_bfs_engine *e, const PolicyArgs &p, hipStream_t s)
+int launch_level(bw_bfs_engine *e, const PolicyArgs &p, hipStream_t s, uint32_t kinds)
 {
-    const uint32_t kinds = policy_kinds(e, p);
     if (kinds & 1u) BW_TRY(launch_step(e, false, s));
     if (kinds & 2u) BW_TRY(launch_step(e, true, s));
     hipLaunchKernelGGL(bfs_engine_epilogue, dim3(1), dim3(kBlock), 0, s, e->g, e->b, p, e->grid);
@@ -1327,7 +1336,8 @@ extern "C" int bw_bfs_engine_level(bw_bfs_engine *e, int32_t policy, int32_t fix
 {
     if (!e) return fail(BW_HIP_ERR_ARG, "engine is NULL");
     BW_TRY(check_policy(e, policy, fixed, schedule_len));
-    return launch_level(e, make_policy(e, policy, fixed, schedule_len), static_cast<hipStream_t>(stream));
+    const PolicyArgs p = make_policy(e, policy, fixed, schedule_len);
+    return launch_level(e, p, static_cast<hipStream_t>(stream), policy_kinds(e, p));
 }
 
 // Overrides the variant of the pending level.
@@ -1457,6 +1467,14 @@ int run_impl(bw_bfs_engine *e, uint32_t root, const uint32_t *roots_host, uint32
     }
     if (roots_host) BW_TRY(upload_roots(e, roots_host, nroots, s));
     const PolicyArgs p = make_policy(e, policy, fixed, schedule_len);
+    // A learned policy over both kinds launches only the native level kernel
+    // (the reference families' kernel would mostly return at once, a fixed
+    // cost per level) until a level picks a reference family.  That level's
+    // epilogue leaves it pending and flags the state as stalled, and from the
+    // next batch on the search launches both kinds.
+    const uint32_t all_kinds = policy_kinds(e, p);
+    const uint32_t graph_kinds = (policy == LEARNED && all_kinds == 3u) ? 1u : all_kinds;
+    uint32_t kinds = graph_kinds;
     if (use_graph) {
         const int32_t key[4] = {policy, fixed, std::max(schedule_len, 1), levels_per_sync};
         if (e->exec && (std::memcmp(key, e->exec_key, sizeof key) != 0 || e->exec_stream != s)) {
@@ -1468,7 +1486,8 @@ int run_impl(bw_bfs_engine *e, uint32_t root, const uint32_t *roots_host, uint32
             hipGraph_t graph = nullptr;
             BW_HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
             int cst = BW_HIP_OK;
-            for (int32_t i = 0; cst == BW_HIP_OK && i < levels_per_sync; i++) cst = launch_level(e, p, s);
+            for (int32_t i = 0; cst == BW_HIP_OK && i < levels_per_sync; i++)
+                cst = launch_level(e, p, s, graph_kinds);
             hipError_t ce = hipStreamEndCapture(s, &graph);
             if (cst != BW_HIP_OK) {
                 if (graph) (void)hipGraphDestroy(graph);
@@ -1492,24 +1511,27 @@ int run_impl(bw_bfs_engine *e, uint32_t root, const uint32_t *roots_host, uint32
     if (hipEventRecord(ev0, s) != hipSuccess) st = fail(BW_HIP_ERR_RUNTIME, "event record failed");
     if (st == BW_HIP_OK)
         st = begin_impl(e, root, roots_host ? e->roots : nullptr, nroots, policy, fixed, schedule_len, s);
-    uint32_t done = 0;
+    BfsState bs{};
     uint32_t batches = 0;
-    const uint32_t max_batches = e->b.max_levels / static_cast<uint32_t>(levels_per_sync) + 2;
-    while (st == BW_HIP_OK && !done) {
+    // One more batch than the levels need: the one a stall leaves unfinished.
+    const uint32_t max_batches = e->b.max_levels / static_cast<uint32_t>(levels_per_sync) + 3;
+    while (st == BW_HIP_OK && !bs.done) {
         if (batches++ > max_batches) {
             st = fail(BW_HIP_ERR_LIMIT, "BFS engine exceeded its level bound");
             break;
         }
-        if (use_graph) {
+        if (use_graph && kinds == graph_kinds) {
             hipError_t le = hipGraphLaunch(e->exec, s);
             if (le != hipSuccess) st = hip_fail(le, "hipGraphLaunch", __FILE__, __LINE__);
         } else {
-            for (int32_t i = 0; st == BW_HIP_OK && i < levels_per_sync; i++) st = launch_level(e, p, s);
+            for (int32_t i = 0; st == BW_HIP_OK && i < levels_per_sync; i++) st = launch_level(e, p, s, kinds);
         }
         if (st != BW_HIP_OK) break;
-        if (hipMemcpyAsync(&done, &e->b.st->done, sizeof done, hipMemcpyDeviceToHost, s) != hipSuccess ||
+        if (hipMemcpyAsync(&bs, e->b.st, sizeof bs, hipMemcpyDeviceToHost, s) != hipSuccess ||
             hipStreamSynchronize(s) != hipSuccess)
             st = fail(BW_HIP_ERR_RUNTIME, "reading the BFS state failed");
+        else if (bs.stalled)
+            kinds = all_kinds;
     }
     if (st == BW_HIP_OK) {
         if (hipEventRecord(ev1, s) != hipSuccess || hipEventSynchronize(ev1) != hipSuccess)

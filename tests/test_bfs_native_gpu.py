@@ -260,6 +260,46 @@ def test_edge_list_variants_need_their_representation(gpu):
         dg.free()
 
 
+def _depth_model(ref_variant: str, from_depth: int) -> np.ndarray:
+    """An 11-output model that predicts `td` fastest below `from_depth` and
+    the reference family `ref_variant` from there on: hidden unit 0 carries
+    the depth feature through both layers, `td`'s output is 5, the
+    family's 5 + 1.5 * (from_depth - 0.5 - depth), every other 10."""
+    K, F, H = len(bfs_native.VARIANTS), bfs_native.NUM_FEATURES, switch.HIDDEN
+    L = switch._layout(K)
+    w = np.zeros(L["end"], dtype=np.float32)
+    w[L["std"]:L["std"] + F] = 1.0
+    w[L["W1"] + 0 * F + 3] = 1.0   # feature 3 is the depth
+    w[L["W2"] + 0 * H + 0] = 1.0
+    k = bfs_native.variant_id(ref_variant)
+    w[L["W3"] + k * H + 0] = -1.5
+    w[L["b3"]:L["end"]] = 10.0
+    w[L["b3"] + bfs_native.variant_id("td")] = 5.0
+    w[L["b3"] + k] = 5.0 + 1.5 * (from_depth - 0.5)
+    return w
+
+
+@pytest.mark.parametrize("use_graph", [True, False])
+@pytest.mark.parametrize("ref_variant,from_depth", [("vertex-push", 2), ("edge-list", 3), ("vertex-pull-warp", 1)])
+def test_learned_launches_reference_kernel_when_picked(cases, use_graph, ref_variant, from_depth):
+    """The learned policy launches only the native level kernel until a level
+    picks a reference family; that level stalls and the search goes on with
+    both kinds.  Depths and counts match the oracle, and every level ran the
+    variant the model picked for it."""
+    eng_model = _depth_model(ref_variant, from_depth)
+    for g, csr, dg, eng, roots in cases[:4]:
+        eng.set_model(eng_model)
+        for root in roots:
+            print(f"case {g.name} root={root} {ref_variant} from depth {from_depth} graph={use_graph}", flush=True)
+            want, counts = _want(csr, root)
+            for _ in range(2):
+                r = eng.run(root, "learned", "td", use_graph=use_graph, levels_per_sync=4)
+                assert np.array_equal(r.levels, want) and np.array_equal(r.counts, counts), (g.name, root)
+                expect = [bfs_native.variant_id("td") if d < from_depth else bfs_native.variant_id(ref_variant)
+                          for d in range(r.num_levels)]
+                assert r.choices.tolist() == expect, (g.name, root, r.choices.tolist())
+
+
 def test_native_model_still_loads(cases):
     """A model with one output per native variant picks among those."""
     g, csr, dg, eng, roots = cases[0]
