@@ -54,6 +54,8 @@ def lib() -> C.CDLL:
         L.oracle_pagerank_f32.argtypes = [G, C.c_int, C.c_int, C.c_int, C.c_uint32, C.c_void_p, C.c_void_p]
         L.oracle_pagerank_f64.restype = None
         L.oracle_pagerank_f64.argtypes = [G, C.c_int, C.c_int, C.c_int, C.c_void_p]
+        L.oracle_pagerank_binned_fixed.restype = None
+        L.oracle_pagerank_binned_fixed.argtypes = [G, C.c_int, C.c_int, C.c_void_p]
         L.oracle_pagerank_pull_omp.restype = None
         L.oracle_pagerank_pull_omp.argtypes = [G, C.c_int, C.c_void_p, C.c_int, C.POINTER(C.c_double)]
         L.oracle_pagerank_bucket_order.restype = None
@@ -147,6 +149,14 @@ def pagerank_f64(csr: CSR, iterations: int, mode: str = "drop", nodiv_quirk: boo
     x = np.empty(max(csr.V, 1), dtype=np.float64)
     lib().oracle_pagerank_f64(csr.ref, iterations, MODES[mode], int(nodiv_quirk), x.ctypes.data)
     return x[:csr.V]
+
+
+def pagerank_binned_fixed(csr: CSR, iterations: int, nodiv_quirk: bool = False) -> np.ndarray:
+    """DROP-mode ranks with the binned iteration's fixed-point row sums
+    (oracle_pagerank_binned_fixed)."""
+    r = np.empty(max(csr.V, 1), dtype=np.float32)
+    lib().oracle_pagerank_binned_fixed(csr.ref, iterations, int(nodiv_quirk), r.ctypes.data)
+    return r[:csr.V]
 
 
 def pagerank_pull_omp(csr: CSR, iterations: int, threads: int = 0, timed: bool = False):

@@ -474,6 +474,47 @@ void oracle_pagerank_f64(const oracle_graph *g, int iterations, int mode,
     free(nx);
 }
 
+void oracle_pagerank_binned_fixed(const oracle_graph *g, int iterations, int nodiv_quirk, float *ranks)
+{
+    const uint64_t V = g->V;
+    if (V == 0) return;
+    uint32_t *deg = (uint32_t *)malloc(sizeof(uint32_t) * V);
+    float *c = (float *)malloc(sizeof(float) * V);
+    float *nc = (float *)malloc(sizeof(float) * V);
+    oracle_pr_out_degrees(g, deg);
+    int lg = 0;
+    if (nodiv_quirk) {
+        const double M = 1.0 + (double)g->E / (double)V;
+        while (ldexp(1.0, lg) < M) lg++;
+    }
+    const int S = 62 - lg;
+    const float init = 1.0f / (float)V;
+    const float base = (1.0f - DAMPENING) / (float)V;
+    /* Initial contributions: 1/V, divided by the out-degree unless the
+     * first iteration is undivided (pagerank/pagerank.cpp:88). */
+    for (uint64_t v = 0; v < V; v++) c[v] = (!nodiv_quirk && deg[v]) ? init / (float)deg[v] : init;
+    for (int it = 1; it <= iterations; it++) {
+#pragma omp parallel for schedule(dynamic, 1024)
+        for (int64_t v = 0; v < (int64_t)V; v++) {
+            uint64_t acc = 0;
+            for (uint64_t j = g->roff[v]; j < g->roff[v + 1]; j++)
+                acc += (uint64_t)ldexp((double)c[g->rcol[j]], S);  /* exact product, truncated */
+            const float sum = ldexpf((float)acc, -S);                /* one rounding to FP32 */
+            const float nr = fmaf(DAMPENING, sum, base);
+            ranks[v] = nr;
+            nc[v] = deg[v] ? nr / (float)deg[v] : 0.0f;
+        }
+        float *t = c;
+        c = nc;
+        nc = t;
+    }
+    if (iterations < 1)
+        for (uint64_t v = 0; v < V; v++) ranks[v] = init;
+    free(deg);
+    free(c);
+    free(nc);
+}
+
 void oracle_pagerank_pull_omp(const oracle_graph *g, int iterations, float *pr,
                               int threads, double *seconds)
 {

@@ -118,6 +118,18 @@ void oracle_pagerank_f32(const oracle_graph *g, int variant, int iterations,
 void oracle_pagerank_f64(const oracle_graph *g, int iterations, int mode,
                          int nodiv_quirk, double *ranks);
 
+/* The binned iteration's arithmetic, restated to check it bit for bit:
+ * vertex-pull-nodiv in DROP mode (pagerank/vertex_pull_nodiv.cu:3-29,
+ * consolidateRankNoDiv pagerank/pagerank.cu:142-165) with each vertex's
+ * in-edge sum taken in 64-bit fixed point, sum of floor(c_u * 2^S) over
+ * its in-edges (every stored edge, repeats included), rounded once to FP32
+ * and scaled by 2^-S; new rank fmaf(0.85f, sum, (1 - 0.85f) / V), the
+ * contribution rank / outdeg (0 without out-edges).  S = 62 - ceil(log2 M),
+ * M = 1, or 1 + E/V with the undivided first iteration (nodiv_quirk != 0).
+ * This is what the device computes when each partition is a single chunk.
+ * ranks[V] receives the final ranks. */
+void oracle_pagerank_binned_fixed(const oracle_graph *g, int iterations, int nodiv_quirk, float *ranks);
+
 /* Baseline timing port: config-mode vertex-pull-nodiv over `threads`
  * OpenMP threads; ranks identical to oracle_pagerank_f32 with
  * ORACLE_PR_VERTEX_PULL_NODIV and ORACLE_PR_DANGLING_UNIFORM. */

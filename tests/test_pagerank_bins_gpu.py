@@ -284,3 +284,46 @@ def test_device_relabel_pipeline_ranks(gpu):
             _check(g, csr, ranks, resid, mode, quirk, iters)
     finally:
         fp.close()
+
+
+@pytest.mark.parametrize("rows_log2,lanes", [(13, 8), (14, 8), (13, 16), (14, 16)])
+@pytest.mark.parametrize("quirk", [False, True])
+def test_binned_drop_ranks_bit_exact(cases, rows_log2, lanes, quirk):
+    """With every partition in one chunk (4096 tasks of 2048 edges), a
+    DROP-mode run's ranks are fixed by the fixed-point arithmetic alone:
+    they equal oracle_pagerank_binned_fixed bit for bit, so a single edge
+    added twice or missed anywhere would show."""
+    for g, csr, dg in cases:
+        print(f"case {g.name} bit-exact binned {rows_log2}/{lanes} quirk={quirk}", flush=True)
+        fp = pagerank.FusedPageRank(dg, max_iterations=20)
+        fp.bin(rows_log2, 4096, lanes)
+        parts = -(-g.vertex_count // (1 << rows_log2))
+        assert fp.bin_info()["chunks"] <= parts
+        for iters in (1, 2, 20):
+            ranks, _, _ = fp.run(iters, "drop", nodiv_quirk=quirk)
+            want = O.pagerank_binned_fixed(csr, iters, nodiv_quirk=quirk)
+            bad = np.flatnonzero(ranks.view(np.uint32) != want.view(np.uint32))
+            assert bad.size == 0, (g.name, iters, bad[:5], ranks[bad[:5]], want[bad[:5]])
+        fp.close()
+
+
+@pytest.mark.parametrize("quirk", [False, True])
+def test_binned_drop_ranks_bit_exact_scale_20(gpu, quirk):
+    """R-MAT scale 20 (edge factor 16) relabelled by degree on the device,
+    2^13-row partitions of one chunk each (the hub partition holds 7.4 M of
+    the 16.1 M edges): bit for bit against the oracle's fixed-point
+    restatement after 1 and 10 iterations."""
+    g = graph.rmat(20, 16, seed=7)
+    fp = pagerank.FusedPageRank.for_graph(g, max_iterations=10, reorder=True, on_device=True)
+    try:
+        fp.bin(13, 4096, 8)
+        assert fp.bin_info()["chunks"] <= g.vertex_count >> 13
+        rg, _ = g.reorder_by_degree()
+        csr = O.CSR.from_graph(rg)
+        for iters in (1, 10):
+            ranks, _, _ = fp.run(iters, "drop", nodiv_quirk=quirk)  # in the original order
+            want = O.pagerank_binned_fixed(csr, iters, nodiv_quirk=quirk)[fp.perm]
+            bad = np.flatnonzero(ranks.view(np.uint32) != want.view(np.uint32))
+            assert bad.size == 0, (iters, bad.size, bad[:5], ranks[bad[:5]], want[bad[:5]])
+    finally:
+        fp.close()
