@@ -200,6 +200,7 @@ class FusedPageRank:
         from .device import device_props
         cus = device_props(0)["compute_units"]
         cands = candidates or [("bins", 14, 128, 8), ("bins", 13, 64, 8), ("bins", 13, 32, 8), ("bins", 14, 64, 8),
+                               ("bins", 14, 80, 8), ("bins", 14, 96, 8),
                                (0, 3 * cus, 1, 2, 16), (0, 2 * cus, 1, 4, 16)]
         table = {}
         for shape in cands:

@@ -327,3 +327,22 @@ def test_binned_drop_ranks_bit_exact_scale_20(gpu, quirk):
             assert bad.size == 0, (iters, bad.size, bad[:5], ranks[bad[:5]], want[bad[:5]])
     finally:
         fp.close()
+
+
+def test_autotuned_shape_matches_oracle(gpu):
+    """The autotuner times every candidate shape (binned and pull) on
+    R-MAT scale 18 in both vertex orders and keeps one; the ranks of the
+    kept shape match the oracle."""
+    g = graph.rmat(18, 16, seed=67)
+    csr = O.CSR.from_graph(g)
+    for reorder in (True, False):
+        fp = pagerank.FusedPageRank.for_graph(g, max_iterations=20, reorder=reorder, on_device=True)
+        try:
+            table = fp.autotune()
+            print(f"autotune reorder={reorder}: {table} -> {fp.kernel_names()}", flush=True)
+            assert len(table) >= 6
+            for mode, quirk, iters in (("uniform", False, 20), ("drop", True, 20)):
+                ranks, resid, _ = fp.run(iters, mode, nodiv_quirk=quirk)
+                _check(g, csr, ranks, resid, mode, quirk, iters)
+        finally:
+            fp.close()
