@@ -840,9 +840,22 @@ __global__ __launch_bounds__(kBlock) void bfs_engine_epilogue(Graph g, Buffers b
     if (wid == 0) load_lane_model(p, lm);
     uint64_t os = 0;
     double sq = 0.0;
-    for (uint32_t i = tid; i < nparts; i += kBlock) {
-        os += b.part[2 * i];
-        sq += __longlong_as_double(static_cast<long long>(b.part[2 * i + 1]));
+    // Four partials per thread in flight, added in the order of a plain
+    // stride loop.
+    for (uint32_t i0 = tid; i0 < nparts; i0 += 4 * kBlock) {
+        uint64_t pa[4], pq[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const uint32_t i = i0 + j * kBlock;
+            pa[j] = i < nparts ? b.part[2 * i] : 0;
+            pq[j] = i < nparts ? b.part[2 * i + 1] : 0;
+        }
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            if (i0 + j * kBlock >= nparts) break;
+            os += pa[j];
+            sq += __longlong_as_double(static_cast<long long>(pq[j]));
+        }
     }
     for (int o = kWave / 2; o > 0; o >>= 1) {
         os += __shfl_xor(os, o, kWave);
@@ -858,33 +871,33 @@ __global__ __launch_bounds__(kBlock) void bfs_engine_epilogue(Graph g, Buffers b
     const bool deferred = defer_depths(st, b, g.V);  // this level's, before the state moves on
     __syncthreads();
 
-    // Hub degree prefix of the next frontier (hub_cap / kBlock consecutive
-    // hubs per thread), read by the next top-down level.
+    // Hub degree prefix of the next frontier, read by the next top-down
+    // level: rounds of kBlock consecutive hubs, one per thread, each an
+    // exclusive block scan added to the previous rounds' total (a level
+    // with few hubs takes one round of two dependent loads).
     {
-        const uint32_t per = b.hub_cap / kBlock;
         const uint32_t nq = (d & 1) ^ 1;
-        uint64_t local = 0;
-        for (uint32_t k = 0; k < per; k++) {
-            const uint32_t h = tid * per + k;
-            local += h < nhub ? out_degree(g, b.hubq[nq][h]) : 0;
+        uint64_t carry = 0;
+        for (uint32_t r0 = 0; r0 < nhub; r0 += kBlock) {
+            const uint32_t h = r0 + tid;
+            const uint64_t deg = h < nhub ? out_degree(g, b.hubq[nq][h]) : 0;
+            uint64_t incl = deg;
+            for (int o = 1; o < kWave; o <<= 1) {
+                const uint64_t y = __shfl_up(incl, static_cast<unsigned>(o), kWave);
+                if (lane >= o) incl += y;
+            }
+            if (lane == kWave - 1) hscan[wid] = incl;
+            __syncthreads();
+            uint64_t before = carry, total = carry;
+            for (int w = 0; w < kBlock / kWave; w++) {
+                if (w < wid) before += hscan[w];
+                total += hscan[w];
+            }
+            if (h < nhub) b.hub_prefix[h] = before + incl - deg;
+            carry = total;
+            __syncthreads();  // hscan is rewritten by the next round
         }
-        // Exclusive scan of the threads' sums: within each wave by
-        // shuffles, then the preceding waves' totals.
-        uint64_t incl = local;
-        for (int o = 1; o < kWave; o <<= 1) {
-            const uint64_t y = __shfl_up(incl, static_cast<unsigned>(o), kWave);
-            if (lane >= o) incl += y;
-        }
-        if (lane == kWave - 1) hscan[wid] = incl;
-        __syncthreads();
-        uint64_t run = incl - local;
-        for (int w = 0; w < wid; w++) run += hscan[w];
-        for (uint32_t k = 0; k < per; k++) {
-            const uint32_t h = tid * per + k;
-            if (h <= nhub) b.hub_prefix[h] = run;
-            run += h < nhub ? out_degree(g, b.hubq[nq][h]) : 0;
-        }
-        if (tid == kBlock - 1 && nhub == b.hub_cap) b.hub_prefix[b.hub_cap] = run;
+        if (tid == 0) b.hub_prefix[nhub] = carry;
     }
 
     if (tid == 0) {
