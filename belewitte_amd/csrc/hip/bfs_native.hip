@@ -877,10 +877,22 @@ __global__ __launch_bounds__(kBlock) void bfs_engine_epilogue(Graph g, Buffers b
     // with few hubs takes one round of two dependent loads).
     {
         const uint32_t nq = (d & 1) ^ 1;
+        // The degrees of the first kPreRounds rounds are loaded before any
+        // scan, so those rounds wait on memory once.
+        constexpr int kPreRounds = 16;
+        uint32_t ids[kPreRounds];
+        uint64_t degs[kPreRounds];
+#pragma unroll
+        for (int r = 0; r < kPreRounds; r++) {
+            const uint32_t h = r * kBlock + tid;
+            ids[r] = h < nhub ? b.hubq[nq][h] : 0u;
+        }
+#pragma unroll
+        for (int r = 0; r < kPreRounds; r++)
+            degs[r] = r * kBlock + tid < nhub ? out_degree(g, ids[r]) : 0;
         uint64_t carry = 0;
-        for (uint32_t r0 = 0; r0 < nhub; r0 += kBlock) {
+        auto round = [&](uint32_t r0, uint64_t deg) {
             const uint32_t h = r0 + tid;
-            const uint64_t deg = h < nhub ? out_degree(g, b.hubq[nq][h]) : 0;
             uint64_t incl = deg;
             for (int o = 1; o < kWave; o <<= 1) {
                 const uint64_t y = __shfl_up(incl, static_cast<unsigned>(o), kWave);
@@ -896,6 +908,13 @@ __global__ __launch_bounds__(kBlock) void bfs_engine_epilogue(Graph g, Buffers b
             if (h < nhub) b.hub_prefix[h] = before + incl - deg;
             carry = total;
             __syncthreads();  // hscan is rewritten by the next round
+        };
+#pragma unroll
+        for (int r = 0; r < kPreRounds; r++)
+            if (r * kBlock < nhub) round(r * kBlock, degs[r]);
+        for (uint32_t r0 = kPreRounds * kBlock; r0 < nhub; r0 += kBlock) {
+            const uint32_t h = r0 + tid;
+            round(r0, h < nhub ? out_degree(g, b.hubq[nq][h]) : 0);
         }
         if (tid == 0) b.hub_prefix[nhub] = carry;
     }
