@@ -201,12 +201,10 @@ __global__ __launch_bounds__(kBlock) void msbfs_level(MsArgs a)
 // contiguous chunk of vertices: it counts its entries and edges, reserves
 // them with one atomic, then writes them, so within the queue the reserved
 // ranges increase with the entry.
-__global__ __launch_bounds__(kBlock) void msbfs_td_scan(MsArgs a)
+__device__ __forceinline__ void td_scan(const MsArgs &a, MsState &st)
 {
     __shared__ uint32_t wcnt[kBlock / kWave], wdeg[kBlock / kWave];
     __shared__ unsigned long long base_slot;
-    MsState &st = *a.st;
-    if (st.done || !st.topdown) return;
     const uint64_t *fin = (st.depth & 1) ? a.front[1] : a.front[0];
     const int lane = __lane_id(), wid = threadIdx.x / kWave;
     const uint64_t chunk = (a.V + gridDim.x - 1) / gridDim.x;
@@ -302,10 +300,8 @@ __global__ __launch_bounds__(kBlock) void msbfs_td_expand(MsArgs a)
 // Bottom-up, hubs: one wave per chunk of a hub's in-edges ORs their
 // frontier masks (stopping once the hub's missing searches are covered,
 // counting what other chunks already found) into next[hub].
-__global__ __launch_bounds__(kBlock) void msbfs_bu_hubs(MsArgs a)
+__device__ __forceinline__ void bu_hubs(const MsArgs &a, const MsState &st)
 {
-    const MsState &st = *a.st;
-    if (st.done || st.topdown) return;
     const uint64_t *fin = (st.depth & 1) ? a.front[1] : a.front[0];
     const int lane = __lane_id();
     const uint32_t nwaves = gridDim.x * (kBlock / kWave);
@@ -327,6 +323,18 @@ __global__ __launch_bounds__(kBlock) void msbfs_bu_hubs(MsArgs a)
         const uint64_t nb = acc & ~vis;
         if (lane == 0 && (nb & ~a.next[h.x])) atomicOr(reinterpret_cast<unsigned long long *>(&a.next[h.x]), nb);
     }
+}
+
+// The first kernel of a level, whichever its direction: the top-down
+// queue (td_scan) or the bottom-up hub chunks (bu_hubs).  One launch, where
+// two kernels would each return at once on the levels of the other
+// direction.
+__global__ __launch_bounds__(kBlock) void msbfs_scan_or_hubs(MsArgs a)
+{
+    MsState &st = *a.st;
+    if (st.done) return;
+    if (st.topdown) td_scan(a, st);
+    else if (a.nhubs) bu_hubs(a, st);
 }
 
 // Closes level d; picks the direction of level d + 1 from its frontier's
@@ -466,14 +474,10 @@ int build_hubs(bw_msbfs *m, uint32_t hub_degree)
 int launch_levels(const bw_msbfs *m, const MsArgs &a, int32_t count, hipStream_t s)
 {
     for (int32_t i = 0; i < count; i++) {
-        hipLaunchKernelGGL(msbfs_td_scan, dim3(m->grid), dim3(kBlock), 0, s, a);
+        hipLaunchKernelGGL(msbfs_scan_or_hubs, dim3(m->grid), dim3(kBlock), 0, s, a);
         BW_LAUNCH_CHECK();
         hipLaunchKernelGGL(msbfs_td_expand, dim3(m->grid), dim3(kBlock), 0, s, a);
         BW_LAUNCH_CHECK();
-        if (m->nhubs) {
-            hipLaunchKernelGGL(msbfs_bu_hubs, dim3(m->grid), dim3(kBlock), 0, s, a);
-            BW_LAUNCH_CHECK();
-        }
         hipLaunchKernelGGL(msbfs_level, dim3(m->grid), dim3(kBlock), 0, s, a);
         BW_LAUNCH_CHECK();
         hipLaunchKernelGGL(msbfs_epilogue, dim3(1), dim3(1), 0, s, a);
