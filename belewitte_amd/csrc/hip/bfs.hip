@@ -91,6 +91,8 @@ __global__ __launch_bounds__(1024) void bfs_struct_edge_list_kernel(
 // Out-degree (in-degree for pull) above which a vertex's edges are taken
 // by its whole wave instead of its own thread.
 constexpr uint64_t kWaveDegree = 64;
+constexpr int kPushBatch = 4;                 // edges per lane in flight in the push kernels
+constexpr uint32_t kNoVertex = 0xFFFFFFFFu;   // an empty slot of such a batch
 
 // The 64-bit offset of lane `l` broadcast to the wave.
 __device__ __forceinline__ uint64_t read_lane_u64(uint64_t x, int l)
@@ -128,15 +130,33 @@ __global__ __launch_bounds__(1024) void bfs_vertex_push_kernel(
             e = load_off(off, v + 1);
         }
         const bool big = active && e - s > kWaveDegree;
+        // kPushBatch edges per lane in flight: their column loads are issued
+        // before any depth is probed.  A vertex is counted by the atomic that
+        // lowers it, whatever the order, so depths and counts are unchanged.
         for (uint64_t m = __ballot(big); m; m &= m - 1) {
             const int l = __builtin_ctzll(m);
             const uint64_t hs = read_lane_u64(s, l), he = read_lane_u64(e, l);
-            for (uint64_t j = hs + lane; j < he; j += kWave)
-                if (relax_depth(levels, cols[j], nd)) fc.update(frontier);
+            for (uint64_t j0 = hs + lane; j0 < he; j0 += kWave * kPushBatch) {
+                uint32_t w[kPushBatch];
+#pragma unroll
+                for (int u = 0; u < kPushBatch; u++) {
+                    const uint64_t j = j0 + u * kWave;
+                    w[u] = j < he ? cols[j] : kNoVertex;
+                }
+#pragma unroll
+                for (int u = 0; u < kPushBatch; u++)
+                    if (w[u] != kNoVertex && relax_depth(levels, w[u], nd)) fc.update(frontier);
+            }
         }
         if (active && !big)
-            for (uint64_t j = s; j < e; j++)
-                if (relax_depth(levels, cols[j], nd)) fc.update(frontier);
+            for (uint64_t j0 = s; j0 < e; j0 += kPushBatch) {
+                uint32_t w[kPushBatch];
+#pragma unroll
+                for (int u = 0; u < kPushBatch; u++) w[u] = j0 + u < e ? cols[j0 + u] : kNoVertex;
+#pragma unroll
+                for (int u = 0; u < kPushBatch; u++)
+                    if (w[u] != kNoVertex && relax_depth(levels, w[u], nd)) fc.update(frontier);
+            }
     }
     fc.finalise(frontier);
 }
