@@ -35,6 +35,7 @@
 #include <algorithm>
 #include <cmath>
 #include <thread>
+#include <type_traits>
 #include <vector>
 
 namespace bw {
@@ -544,12 +545,35 @@ inline int bins_fixed_shift(uint64_t V, uint64_t E, bool undivided_first)
     return 62 - lg;
 }
 
+// v moved across lanes by the DPP control kCtrl in the rows of kRowMask;
+// 0 in the other rows (and, with kBound, where the control has no source).
+template<int kCtrl, int kRowMask, bool kBound>
+__device__ __forceinline__ uint32_t dpp_u32(uint32_t v)
+{
+    return static_cast<uint32_t>(__builtin_amdgcn_update_dpp(0, static_cast<int>(v), kCtrl, kRowMask, 0xF, kBound));
+}
+
+// Inclusive prefix sum over the wave's 64 lanes by DPP row shifts (within
+// each row of 16 lanes) and row broadcasts (lane 15 into row 1, lane 47 into
+// row 3, then lane 31 into rows 2 and 3): six VALU adds, where a prefix by
+// ballots takes about six instructions per bit of the summed value.
+__device__ __forceinline__ uint32_t wave_inclusive_scan(uint32_t x)
+{
+    x += dpp_u32<0x111, 0xF, true>(x);   // row_shr:1 (lanes without a source read 0)
+    x += dpp_u32<0x112, 0xF, true>(x);   // row_shr:2
+    x += dpp_u32<0x114, 0xF, true>(x);   // row_shr:4
+    x += dpp_u32<0x118, 0xF, true>(x);   // row_shr:8
+    x += dpp_u32<0x142, 0xA, false>(x);  // row_bcast:15 into rows 1 and 3
+    x += dpp_u32<0x143, 0xC, false>(x);  // row_bcast:31 into rows 2 and 3
+    return x;
+}
+
 // Workgroup per chunk; the last workgroup reduces the previous finalise's
 // (residual, dangling) partials.  Each wave takes whole tasks; per round a
 // lane holds kLaneEdges consecutive edge codes (16-byte loads, the next
 // round's already in flight), finds the pair of each edge from the
-// new-pair flags (wave prefix over per-lane flag counts, by ballots of
-// their bits), reads the pairs' values and adds each to its row.
+// new-pair flags (DPP wave prefix over the per-lane flag counts), reads
+// the pairs' values and adds each to its row.
 template<int kRowsLog, int kThreads, int kLaneEdges, int kDepth>
 __global__ __launch_bounds__(kThreads, kRowsLog == 13 ? 8 : 4) void pr_bins_accumulate_kernel(BinArgs a)
 {
@@ -557,7 +581,6 @@ __global__ __launch_bounds__(kThreads, kRowsLog == 13 ? 8 : 4) void pr_bins_accu
     constexpr int kWaves = kThreads / kWave;
     constexpr int kVec = kLaneEdges / 8;                 // 16-byte loads per lane and round
     constexpr uint32_t kRound = kWave * kLaneEdges;      // edges per wave and round
-    constexpr int kCountBits = kLaneEdges == 8 ? 4 : 5;  // flag count 0..kLaneEdges
     static_assert(kWaves == static_cast<int>(bins::kChunkWaves), "the plan makes one task per wave");
     __shared__ unsigned long long acc[kRows];
     __shared__ uint4 stask[bins::kChunkWaves];  // the chunk's task records
@@ -573,7 +596,6 @@ __global__ __launch_bounds__(kThreads, kRowsLog == 13 ? 8 : 4) void pr_bins_accu
     }
     const int lane = threadIdx.x & (kWave - 1);
     const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
-    const uint64_t below = (1ull << lane) - 1ull;
     // The lane's codes of the round starting at edge e0 of a task ending at
     // `end`.  The loads are unconditional (no branch, so the compiler can
     // keep several rounds in flight): a group past the task's end reads the
@@ -640,8 +662,9 @@ __global__ __launch_bounds__(kThreads, kRowsLog == 13 ? 8 : 4) void pr_bins_accu
             // 125 to 108 us with it).
             constexpr bool kWindow = kLaneEdges == 8;
             auto stage = [&](const u32x4 (&raw)[kVec], uint32_t (&cd)[kLaneEdges / 2], float (&vv)[kLaneEdges],
-                             uint32_t &nn, uint32_t &fl) {
+                             uint32_t &nn, uint32_t &fl, bool &full) {
                 const uint32_t e = se + lane * kLaneEdges;
+                full = se + kRound <= tk.y;  // wave-uniform: every lane has kLaneEdges edges
                 nn = e < tk.y ? min(static_cast<uint32_t>(kLaneEdges), tk.y - e) : 0u;
 #pragma unroll
                 for (int q = 0; q < kVec; q++)
@@ -655,13 +678,12 @@ __global__ __launch_bounds__(kThreads, kRowsLog == 13 ? 8 : 4) void pr_bins_accu
                 for (int j = 0; j < kLaneEdges; j++) f |= ((code_of(cd, j) >> 15) & 1u) << j;
                 f &= (1u << nn) - 1u;  // nn <= 16
                 const uint32_t cnt = static_cast<uint32_t>(__popc(f));
-                uint32_t pre = 0, tot = 0;
-#pragma unroll
-                for (int b = 0; b < kCountBits; b++) {
-                    const uint64_t m = __ballot((cnt >> b) & 1u);
-                    pre += static_cast<uint32_t>(__popcll(m & below)) << b;
-                    tot += static_cast<uint32_t>(__popcll(m)) << b;
-                }
+                // The lane's first pair: the flags of the lanes before it
+                // (R-MAT scale 22: 99.5 -> 95.5 us per iteration against a
+                // prefix by ballots of the count's bits).
+                const uint32_t incl = wave_inclusive_scan(cnt);
+                const uint32_t pre = incl - cnt;
+                const uint32_t tot = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(incl), kWave - 1));
                 const uint32_t p0 = base + pre;
                 fl = f;
                 if constexpr (kWindow) {
@@ -703,8 +725,9 @@ __global__ __launch_bounds__(kThreads, kRowsLog == 13 ? 8 : 4) void pr_bins_accu
             // lane onto the same few LDS words.  The mask is applied here, a
             // round after the loads, so that waiting for them never waits for
             // the next round's.
-            auto add = [&](const uint32_t (&cd)[kLaneEdges / 2], const float (&vv)[kLaneEdges], uint32_t nn,
-                           uint32_t fl) {
+            auto add_round = [&](const uint32_t (&cd)[kLaneEdges / 2], const float (&vv)[kLaneEdges], uint32_t nn,
+                                 uint32_t fl, auto masked) {
+                constexpr bool kMasked = decltype(masked)::value;
                 // Window: edge j's pair is window entry o_j, the number of
                 // new-pair flags of edges 1..j (0 <= o_j <= j).  With 2^14 rows
                 // (4 waves per SIMD, registers to spare), w is the window
@@ -734,8 +757,19 @@ __global__ __launch_bounds__(kThreads, kRowsLog == 13 ? 8 : 4) void pr_bins_accu
 #pragma unroll
                         for (int i = 1; i <= j; i++) x = o == static_cast<uint32_t>(i) ? vv[i] : x;
                     }
-                    if (j < static_cast<int>(nn)) atomicAdd(&acc[code_of(cd, j) & (kRows - 1)], bin_fixed(x));
+                    if (!kMasked || j < static_cast<int>(nn))
+                        atomicAdd(&acc[code_of(cd, j) & (kRows - 1)], bin_fixed(x));
                 }
+            };
+            // With 2^14 rows a full round (every lane kLaneEdges edges: all
+            // rounds but a task's last) adds without the per-edge test of the
+            // edge count and its exec-mask branches (95.5 -> 94.5 us per
+            // iteration at R-MAT scale 22).  The 2^13-row kernel, at 8 waves
+            // per SIMD, ran twice as slow with the second copy of the loop.
+            auto add = [&](const uint32_t (&cd)[kLaneEdges / 2], const float (&vv)[kLaneEdges], uint32_t nn,
+                           uint32_t fl, bool full) {
+                if (kRowsLog == 14 && full) add_round(cd, vv, nn, fl, std::false_type{});
+                else add_round(cd, vv, nn, fl, std::true_type{});
             };
             // kDepth rounds of pair values in flight.  Rounds R_0, R_1, ... of
             // the wave; per step r: the codes of R_{r+kDepth+1} are loaded,
@@ -751,6 +785,7 @@ __global__ __launch_bounds__(kThreads, kRowsLog == 13 ? 8 : 4) void pr_bins_accu
             static_assert(kDepth >= 2, "the pipeline keeps at least two rounds in flight");
             u32x4 raw[2][kVec];
             uint32_t code[kDepth][kLaneEdges / 2], n[kDepth], fl[kDepth];
+            bool full[kDepth];      // wave-uniform: the round of the buffer is full
             float v[kDepth][kLaneEdges];
             bool real[kDepth];      // wave-uniform: R_r of the buffer exists
             bool next_real = true;  // the round whose codes were loaded last exists
@@ -759,16 +794,16 @@ __global__ __launch_bounds__(kThreads, kRowsLog == 13 ? 8 : 4) void pr_bins_accu
             for (int i = 0; i < kDepth; i++) {
                 real[i] = next_real;
                 next_real = load_next(raw[(i + 1) & 1]);
-                stage(raw[i & 1], code[i], v[i], n[i], fl[i]);
+                stage(raw[i & 1], code[i], v[i], n[i], fl[i], full[i]);
             }
             for (;;) {
 #pragma unroll
                 for (int i = 0; i < 2 * kDepth; i++) {
                     const int cur = i % kDepth;
                     const bool have = load_next(raw[(i + kDepth + 1) & 1]);
-                    add(code[cur], v[cur], n[cur], fl[cur]);
+                    add(code[cur], v[cur], n[cur], fl[cur], full[cur]);
                     if (!real[(i + 1) % kDepth]) goto wave_done;  // rounds are consecutive
-                    stage(raw[(i + kDepth) & 1], code[cur], v[cur], n[cur], fl[cur]);
+                    stage(raw[(i + kDepth) & 1], code[cur], v[cur], n[cur], fl[cur], full[cur]);
                     real[cur] = next_real;
                     next_real = have;
                 }
