@@ -189,11 +189,18 @@ def msbfs(scale: int, sources: int, wave_degree: int, topdown_divisor: int, hub_
     dg.free()
 
 
-def bfs(scale: int, sources: int, policy: str, edge_lists: bool = True) -> None:
+def bfs(scale: int, sources: int, policy: str, edge_lists: bool = True, kind: str = "rmat") -> None:
     from . import bfs_native, device, graph, switch
     device.require_gpu()
     device.set_device(0)
-    g = graph.rmat(scale, 16, seed=5, undirected=True)
+    if kind == "rmat":
+        g = graph.rmat(scale, 16, seed=5, undirected=True)
+    elif kind == "er":  # the uniform training graph: 2^24 vertices, average degree 16
+        g = graph.erdos_renyi(1 << 24, 1 << 27, seed=3, undirected=True)
+    elif kind == "grid":  # the grid training graph: 4096 x 4096, 1% of its edges rewired
+        g = graph.grid2d(4096, 4096, rewire=0.01, seed=4)
+    else:
+        raise ValueError(f"unknown graph kind {kind!r}")
     dg = device.DeviceGraph(g)
     eng = bfs_native.BFSEngine(dg, edge_lists=edge_lists)
     if policy == "learned":
@@ -246,6 +253,8 @@ def main(argv=None) -> int:
     b.add_argument("--sources", type=int, default=4)
     b.add_argument("--policy", default="learned", help="learned, beamer or a variant name")
     b.add_argument("--no-edge-lists", action="store_true", help="engine without the edge-list families")
+    b.add_argument("--graph", default="rmat", choices=["rmat", "er", "grid"],
+                   help="undirected R-MAT of --scale, or the Erdos-Renyi or grid training graph")
     v = sub.add_parser("pagerank-variants")
     v.add_argument("--scale", type=int, default=22)
     v.add_argument("--iterations", type=int, default=10)
@@ -263,7 +272,7 @@ def main(argv=None) -> int:
         from .pagerank import VARIANTS
         pagerank_variants(a.scale, a.iterations, a.variants.split(",") if a.variants else VARIANTS)
     else:
-        bfs(a.scale, a.sources, a.policy, not a.no_edge_lists)
+        bfs(a.scale, a.sources, a.policy, not a.no_edge_lists, a.graph)
     return 0
 
 
