@@ -120,6 +120,8 @@ def test_cfg2_fused_at_scale_22(cfg2, mode):
     forms.append(("binned degree-ordered", fp, None))
     nat = pagerank.FusedPageRank(dg, max_iterations=30)
     forms.append(("binned generated order", nat, (14, 128, 8)))
+    # the shape the autotuner picks for the generated order (its bench figure)
+    forms.append(("binned generated order, 80-task chunks", nat, (14, 80, 8)))
     forms.append(("pull kernels", nat, 0))
     for label, f, shape in forms:
         if shape is None:
